@@ -1,0 +1,436 @@
+// build.hip -- CSR -> ABHSF conversion on the GPU.
+//
+// The reference computes the storage footprint by sorting entries by block key
+// and walking maximal runs of equal (I,J) (preprocessing/test.cpp:237-368),
+// calling a per-block cost processor (test.cpp:36-108).  Here every step is a
+// data-parallel pass over HBM-resident arrays:
+//   1. validate CSR, expand rows, key = (I << bits(N)) | J, payload = entry id
+//   2. stable LSD radix sort by key -> entries in (I, J, lrow, lcol) order
+//      (CSR order is already (row, col), so stability gives the in-block order
+//      of spmv/orig/spmv.cpp:85-110)
+//   3. head flags + exclusive scan -> block id per entry, block starts
+//   4. per block: z, scheme by the cost model, footprint bits
+//   5. per scheme: exclusive scans of (is scheme X) and (values of X) -> each
+//      block's ordinal and value offset inside its scheme group
+//   6. strip pointers, block metadata, and the entry emit (COO index, CSR row
+//      pointers + columns, bitmap bits by atomicOr, dense positions)
+// All arithmetic is integer, so the result is bit-exact and deterministic.
+#include "storage.hpp"
+
+#include <chrono>
+
+namespace abhsf {
+namespace {
+
+constexpr int kT = 256;
+
+inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+__device__ inline void st_uint(void *p, int bytes, uint64_t i, uint32_t v)
+{
+    if (bytes == 1) static_cast<uint8_t *>(p)[i] = (uint8_t)v;
+    else if (bytes == 2) static_cast<uint16_t *>(p)[i] = (uint16_t)v;
+    else static_cast<uint32_t *>(p)[i] = v;
+}
+
+__device__ inline void st_val(void *p, int bytes, uint64_t i, double v)
+{
+    if (bytes == 8) static_cast<double *>(p)[i] = v;
+    else static_cast<float *>(p)[i] = (float)v;
+}
+
+template <typename T>
+struct Four {
+    T p[4];
+};
+
+__global__ void k_check_rowptr(const int64_t *__restrict__ rp, int64_t m, int64_t nnz, uint32_t *err)
+{
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r > m) return;
+    if (r == 0 && rp[0] != 0) atomicOr(err, 1u);
+    if (r == m && rp[m] != nnz) atomicOr(err, 2u);
+    if (r < m && rp[r + 1] < rp[r]) atomicOr(err, 4u);
+}
+
+__global__ void k_rows_keys(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int64_t m, int64_t n,
+                            uint64_t nnz, int k, int jbits, uint32_t *__restrict__ rows,
+                            uint64_t *__restrict__ key, uint32_t *__restrict__ perm, uint32_t *err)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (e >= nnz) return;
+    int64_t lo = 0, hi = m - 1;  // last row r with rp[r] <= e
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if ((uint64_t)rp[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    const int32_t c = col[e];
+    if (c < 0 || (int64_t)c >= n) atomicOr(err, 8u);
+    if ((int64_t)e > rp[lo] && c <= col[e - 1]) atomicOr(err, 16u);
+    rows[e] = (uint32_t)lo;
+    key[e] = ((uint64_t)((uint64_t)lo >> k) << jbits) | (uint64_t)((uint32_t)c >> k);
+    perm[e] = (uint32_t)e;
+}
+
+__global__ void k_heads(const uint64_t *__restrict__ key, uint64_t nnz, uint32_t *__restrict__ flag)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (e >= nnz) return;
+    flag[e] = (e == 0 || key[e] != key[e - 1]) ? 1u : 0u;
+}
+
+__global__ void k_block_starts(const uint64_t *__restrict__ key, const uint32_t *__restrict__ flag,
+                               const uint32_t *__restrict__ excl, uint64_t nnz, uint32_t nblk,
+                               uint32_t *__restrict__ bstart, uint64_t *__restrict__ bkey)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (e == 0) bstart[nblk] = (uint32_t)nnz;
+    if (e >= nnz || !flag[e]) return;
+    const uint32_t b = excl[e];
+    bstart[b] = (uint32_t)e;
+    bkey[b] = key[e];
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// z, scheme, per-scheme indicator/value counts, footprint sums.
+__global__ void k_block_scheme(const uint32_t *__restrict__ bstart, uint32_t nblk, int s, int policy,
+                               uint64_t cost_b, int is_binary, uint64_t hdr_bits, uint8_t *__restrict__ sch,
+                               Four<uint32_t *> cnt, Four<uint32_t *> ecnt, unsigned long long *totals)
+{
+    const uint32_t b = blockIdx.x * kT + threadIdx.x;
+    unsigned long long pre = 0, msmf = 0, ev[4] = {0, 0, 0, 0};
+    if (b < nblk) {
+        const uint64_t z = bstart[b + 1] - bstart[b];
+        uint64_t bits;
+        const int x = choose_scheme(policy, (uint64_t)s, z, cost_b, is_binary, nullptr);
+        sch[b] = (uint8_t)x;
+        const uint32_t evx = (x == DENSE) ? (uint32_t)(s * s) : (uint32_t)z;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            cnt.p[q][b] = (q == x) ? 1u : 0u;
+            ecnt.p[q][b] = (q == x) ? evx : 0u;
+            ev[q] = (q == x) ? evx : 0u;
+        }
+        choose_scheme(ABHSF_POLICY_PREPROC, (uint64_t)s, z, cost_b, is_binary, &bits);
+        pre = hdr_bits + bits;  // preprocessing/test.cpp:76
+        choose_scheme(ABHSF_POLICY_MSMF, (uint64_t)s, z, cost_b, is_binary, &bits);
+        msmf = bits + 2;        // block_statistics/mmf.cpp:93-96
+    }
+    pre = wave_sum_u64(pre);
+    msmf = wave_sum_u64(msmf);
+#pragma unroll
+    for (int q = 0; q < 4; q++) ev[q] = wave_sum_u64(ev[q]);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&totals[0], pre);
+        atomicAdd(&totals[1], msmf);
+#pragma unroll
+        for (int q = 0; q < 4; q++) atomicAdd(&totals[2 + q], ev[q]);
+    }
+}
+
+__global__ void k_strip_ptrs(const uint64_t *__restrict__ bkey, uint32_t nblk, int jbits, int64_t M,
+                             Four<const uint32_t *> ord, Four<const uint32_t *> eoff, Four<uint32_t> nbx,
+                             Four<uint32_t> nex, Four<uint32_t *> sptr, Four<uint32_t *> eptr)
+{
+    const uint32_t b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nblk) return;
+    const int64_t I = (int64_t)(bkey[b] >> jbits);
+    const int64_t Ip = b ? (int64_t)(bkey[b - 1] >> jbits) : -1;
+    for (int64_t i = Ip + 1; i <= I; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            sptr.p[q][i] = ord.p[q][b];
+            eptr.p[q][i] = eoff.p[q][b];
+        }
+    if (b == nblk - 1)
+        for (int64_t i = I + 1; i <= M; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                sptr.p[q][i] = nbx.p[q];
+                eptr.p[q][i] = nex.p[q];
+            }
+}
+
+__global__ void k_block_meta(const uint64_t *__restrict__ bkey, const uint32_t *__restrict__ bstart,
+                             const uint8_t *__restrict__ sch, uint32_t nblk, uint64_t jmask, int bcol_bytes,
+                             Four<const uint32_t *> ord, Four<void *> bcol, Four<uint16_t *> z1)
+{
+    const uint32_t b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nblk) return;
+    const int x = sch[b];
+    const uint32_t o = ord.p[x][b];
+    st_uint(bcol.p[x], bcol_bytes, o, (uint32_t)(bkey[b] & jmask));
+    z1.p[x][o] = (uint16_t)(bstart[b + 1] - bstart[b] - 1);
+}
+
+struct EmitOut {
+    void *val[4];
+    void *coo_idx;
+    void *csr_rowptr;
+    uint8_t *csr_col;
+    unsigned long long *bm_words;
+    int coo_idx_bytes, csr_rp_bytes, val_bytes, W;
+};
+
+__global__ void k_emit(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ excl,
+                       const uint32_t *__restrict__ bstart, const uint8_t *__restrict__ sch,
+                       const uint32_t *__restrict__ perm, const uint32_t *__restrict__ rows,
+                       const int32_t *__restrict__ col, const double *__restrict__ val, uint64_t nnz, int s, int k,
+                       Four<const uint32_t *> ord, Four<const uint32_t *> eoff, EmitOut out)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (e >= nnz) return;
+    const uint32_t b = excl[e] + flag[e] - 1u;
+    const uint32_t q = (uint32_t)e - bstart[b];
+    const uint32_t z = bstart[b + 1] - bstart[b];
+    const uint32_t el = perm[e];
+    const uint32_t smask = (uint32_t)s - 1u;
+    const uint32_t lr = rows[el] & smask, lc = (uint32_t)col[el] & smask;
+    const double v = val ? val[el] : 1.0;
+    const int x = sch[b];
+    const uint32_t o = ord.p[x][b];
+    const uint64_t eo = eoff.p[x][b];
+    switch (x) {
+    case COO:
+        st_uint(out.coo_idx, out.coo_idx_bytes, eo + q, (lr << k) | lc);
+        st_val(out.val[COO], out.val_bytes, eo + q, v);
+        break;
+    case CSR: {
+        out.csr_col[eo + q] = (uint8_t)lc;
+        st_val(out.val[CSR], out.val_bytes, eo + q, v);
+        const uint64_t rb = (uint64_t)o * (uint64_t)(s + 1);
+        const int prev = (q == 0) ? -1 : (int)(rows[perm[e - 1]] & smask);
+        for (int r = prev + 1; r <= (int)lr; r++) st_uint(out.csr_rowptr, out.csr_rp_bytes, rb + r, q);
+        if (q == z - 1)
+            for (int r = (int)lr + 1; r <= s; r++) st_uint(out.csr_rowptr, out.csr_rp_bytes, rb + r, z);
+        break;
+    }
+    case BITMAP: {
+        const uint32_t p = lr * (uint32_t)s + lc;
+        atomicOr(&out.bm_words[(uint64_t)o * out.W + (p >> 6)], 1ull << (p & 63));
+        st_val(out.val[BITMAP], out.val_bytes, eo + q, v);
+        break;
+    }
+    default:
+        st_val(out.val[DENSE], out.val_bytes, eo + (uint64_t)lr * s + lc, v);
+    }
+}
+
+template <typename T>
+T read_dev(const T *p, hipStream_t st)
+{
+    T h;
+    ABHSF_HIP(hipMemcpyAsync(&h, p, sizeof(T), hipMemcpyDeviceToHost, st));
+    ABHSF_HIP(hipStreamSynchronize(st));
+    return h;
+}
+
+int bits_for(uint64_t maxval)
+{
+    int b = 1;
+    while (b < 64 && (maxval >> b) != 0) b++;
+    return b;
+}
+
+}  // namespace
+
+void Storage::set_shape(int64_t m_, int64_t n_, int s_)
+{
+    m = m_;
+    n = n_;
+    s = s_;
+    k = (int)ceil_log2_u64((uint64_t)s_);
+    M = (m + s - 1) / s;
+    N = (n + s - 1) / s;
+    W = s * s / 64 > 0 ? s * s / 64 : 1;
+    bcol_bytes = N <= 65536 ? 2 : 4;
+    coo_idx_bytes = s <= 16 ? 1 : 2;
+    csr_rp_bytes = s <= 8 ? 1 : 2;
+}
+
+DevMat Storage::view() const
+{
+    DevMat A;
+    A.m = m; A.n = n; A.M = M; A.N = N;
+    A.s = s; A.k = k; A.W = W;
+    for (int x = 0; x < 4; x++) {
+        A.sptr[x] = sptr[x].get<uint32_t>();
+        A.eptr[x] = eptr[x].get<uint32_t>();
+        A.bcol[x] = bcol[x].get();
+        A.z1[x] = z1[x].get<uint16_t>();
+        A.val[x] = val[x].get();
+    }
+    A.coo_idx = coo_idx.get();
+    A.csr_rowptr = csr_rowptr.get();
+    A.csr_col = csr_col.get<uint8_t>();
+    A.bm_words = bm_words.get<uint64_t>();
+    return A;
+}
+
+uint64_t Storage::value_bytes() const
+{
+    uint64_t b = 0;
+    for (int x = 0; x < 4; x++) b += val[x].bytes();
+    return b;
+}
+
+uint64_t Storage::stored_bytes() const
+{
+    uint64_t b = value_bytes() + coo_idx.bytes() + csr_rowptr.bytes() + csr_col.bytes() + bm_words.bytes();
+    for (int x = 0; x < 4; x++) b += sptr[x].bytes() + eptr[x].bytes() + bcol[x].bytes() + z1[x].bytes();
+    return b;
+}
+
+void build_from_device_csr(Storage &S, const int64_t *rowptr, const int32_t *col, const double *val, hipStream_t st)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t nnz = (uint64_t)S.nnz;
+    const int64_t m = S.m, M = S.M;
+    ABHSF_REQUIRE(nnz < (1ull << 32), ABHSF_ERR_UNSUPPORTED, "nnz must be below 2^32 per matrix");
+    const int jbits = bits_for((uint64_t)(S.N > 0 ? S.N - 1 : 0));
+    const int ibits = bits_for((uint64_t)(M > 0 ? M - 1 : 0));
+    ABHSF_REQUIRE(jbits + ibits <= 64, ABHSF_ERR_UNSUPPORTED, "block key does not fit 64 bits");
+
+    DevBuf err(sizeof(uint32_t));
+    ABHSF_HIP(hipMemsetAsync(err.get(), 0, sizeof(uint32_t), st));
+    k_check_rowptr<<<grid_for((uint64_t)m + 1), kT, 0, st>>>(rowptr, m, (int64_t)nnz, err.get<uint32_t>());
+    ABHSF_HIP(hipGetLastError());
+
+    DevBuf rows(nnz * 4), key(nnz * 8), perm(nnz * 4), flag(nnz * 4), excl(nnz * 4);
+    if (nnz) {
+        k_rows_keys<<<grid_for(nnz), kT, 0, st>>>(rowptr, col, m, S.n, nnz, S.k, jbits, rows.get<uint32_t>(),
+                                                  key.get<uint64_t>(), perm.get<uint32_t>(), err.get<uint32_t>());
+        ABHSF_HIP(hipGetLastError());
+    }
+    const uint32_t ecode = read_dev(err.get<uint32_t>(), st);
+    ABHSF_REQUIRE(ecode == 0, ABHSF_ERR_INVALID,
+                  std::string("malformed CSR input (") + ((ecode & 7) ? "row pointers" : "") +
+                      ((ecode & 8) ? " column out of range" : "") +
+                      ((ecode & 16) ? " columns not strictly increasing within a row" : "") + ")");
+
+    radix_sort_pairs(key.get<uint64_t>(), perm.get<uint32_t>(), nnz, jbits + ibits, st);
+
+    uint32_t nblk = 0;
+    DevBuf d_nblk(sizeof(uint32_t));
+    if (nnz) {
+        k_heads<<<grid_for(nnz), kT, 0, st>>>(key.get<uint64_t>(), nnz, flag.get<uint32_t>());
+        ABHSF_HIP(hipGetLastError());
+        exclusive_scan_u32(flag.get<uint32_t>(), excl.get<uint32_t>(), nnz, d_nblk.get<uint32_t>(), st);
+        nblk = read_dev(d_nblk.get<uint32_t>(), st);
+    }
+
+    DevBuf bstart(((uint64_t)nblk + 1) * 4), bkey((uint64_t)nblk * 8), sch((uint64_t)nblk);
+    DevBuf cnt[4], ecnt[4];
+    for (int x = 0; x < 4; x++) {
+        cnt[x].alloc((uint64_t)nblk * 4);
+        ecnt[x].alloc((uint64_t)nblk * 4);
+    }
+    DevBuf totals(6 * sizeof(unsigned long long));
+    ABHSF_HIP(hipMemsetAsync(totals.get(), 0, totals.bytes(), st));
+    unsigned long long htot[6] = {0, 0, 0, 0, 0, 0};
+    if (nnz) {
+        k_block_starts<<<grid_for(nnz), kT, 0, st>>>(key.get<uint64_t>(), flag.get<uint32_t>(),
+                                                     excl.get<uint32_t>(), nnz, nblk, bstart.get<uint32_t>(),
+                                                     bkey.get<uint64_t>());
+        ABHSF_HIP(hipGetLastError());
+        Four<uint32_t *> c4, e4;
+        for (int x = 0; x < 4; x++) { c4.p[x] = cnt[x].get<uint32_t>(); e4.p[x] = ecnt[x].get<uint32_t>(); }
+        const uint64_t hdr = ceil_log2_u64((uint64_t)M) + ceil_log2_u64((uint64_t)S.N) + 2;  // test.cpp:49
+        k_block_scheme<<<grid_for(nblk), kT, 0, st>>>(bstart.get<uint32_t>(), nblk, S.s, S.policy, S.cost_b,
+                                                      S.is_binary, hdr, sch.get<uint8_t>(), c4, e4,
+                                                      totals.get<unsigned long long>());
+        ABHSF_HIP(hipGetLastError());
+        ABHSF_HIP(hipMemcpyAsync(htot, totals.get(), sizeof(htot), hipMemcpyDeviceToHost, st));
+        ABHSF_HIP(hipStreamSynchronize(st));
+    }
+    for (int x = 0; x < 4; x++)
+        ABHSF_REQUIRE(htot[2 + x] < (1ull << 32), ABHSF_ERR_UNSUPPORTED, "stored values of one scheme exceed 2^32");
+    S.smf_preproc = htot[0];
+    // block_statistics/mmf.cpp:68: top-level block CSR of the adaptive format
+    S.smf_msmf = htot[1] + (uint64_t)M * ceil_log2_u64((uint64_t)S.N + 1) + (uint64_t)nblk * ceil_log2_u64((uint64_t)S.N);
+
+    // per-scheme ordinals and value offsets (in place)
+    DevBuf tot4(8 * sizeof(uint32_t));
+    ABHSF_HIP(hipMemsetAsync(tot4.get(), 0, tot4.bytes(), st));
+    for (int x = 0; x < 4; x++) {
+        exclusive_scan_u32(cnt[x].get<uint32_t>(), cnt[x].get<uint32_t>(), nblk, tot4.get<uint32_t>() + x, st);
+        exclusive_scan_u32(ecnt[x].get<uint32_t>(), ecnt[x].get<uint32_t>(), nblk, tot4.get<uint32_t>() + 4 + x, st);
+    }
+    uint32_t h4[8];
+    ABHSF_HIP(hipMemcpyAsync(h4, tot4.get(), sizeof(h4), hipMemcpyDeviceToHost, st));
+    ABHSF_HIP(hipStreamSynchronize(st));
+    for (int x = 0; x < 4; x++) {
+        S.nb[x] = h4[x];
+        S.ne[x] = h4[4 + x];
+    }
+
+    // final storage
+    const int vb = S.val_bytes;
+    for (int x = 0; x < 4; x++) {
+        S.sptr[x].alloc(((uint64_t)M + 1) * 4);
+        S.eptr[x].alloc(((uint64_t)M + 1) * 4);
+        S.bcol[x].alloc(S.nb[x] * (uint64_t)S.bcol_bytes);
+        S.z1[x].alloc(S.nb[x] * 2);
+        S.val[x].alloc(S.ne[x] * (uint64_t)vb);
+    }
+    S.coo_idx.alloc(S.ne[COO] * (uint64_t)S.coo_idx_bytes);
+    S.csr_rowptr.alloc(S.nb[CSR] * (uint64_t)(S.s + 1) * (uint64_t)S.csr_rp_bytes);
+    S.csr_col.alloc(S.ne[CSR]);
+    S.bm_words.alloc(S.nb[BITMAP] * (uint64_t)S.W * 8);
+    if (S.bm_words.bytes()) ABHSF_HIP(hipMemsetAsync(S.bm_words.get(), 0, S.bm_words.bytes(), st));
+    if (S.val[DENSE].bytes()) ABHSF_HIP(hipMemsetAsync(S.val[DENSE].get(), 0, S.val[DENSE].bytes(), st));
+
+    Four<const uint32_t *> ord, eoff;
+    Four<uint32_t> nbx, nex;
+    Four<uint32_t *> sp, ep;
+    Four<void *> bc;
+    Four<uint16_t *> zz;
+    for (int x = 0; x < 4; x++) {
+        ord.p[x] = cnt[x].get<uint32_t>();
+        eoff.p[x] = ecnt[x].get<uint32_t>();
+        nbx.p[x] = (uint32_t)S.nb[x];
+        nex.p[x] = (uint32_t)S.ne[x];
+        sp.p[x] = S.sptr[x].get<uint32_t>();
+        ep.p[x] = S.eptr[x].get<uint32_t>();
+        bc.p[x] = S.bcol[x].get();
+        zz.p[x] = S.z1[x].get<uint16_t>();
+    }
+    if (nblk == 0) {
+        for (int x = 0; x < 4; x++) {
+            ABHSF_HIP(hipMemsetAsync(S.sptr[x].get(), 0, S.sptr[x].bytes(), st));
+            ABHSF_HIP(hipMemsetAsync(S.eptr[x].get(), 0, S.eptr[x].bytes(), st));
+        }
+    } else {
+        k_strip_ptrs<<<grid_for(nblk), kT, 0, st>>>(bkey.get<uint64_t>(), nblk, jbits, M, ord, eoff, nbx, nex, sp, ep);
+        ABHSF_HIP(hipGetLastError());
+        const uint64_t jmask = (jbits >= 64) ? ~0ull : ((1ull << jbits) - 1);
+        k_block_meta<<<grid_for(nblk), kT, 0, st>>>(bkey.get<uint64_t>(), bstart.get<uint32_t>(), sch.get<uint8_t>(),
+                                                    nblk, jmask, S.bcol_bytes, ord, bc, zz);
+        ABHSF_HIP(hipGetLastError());
+        EmitOut out;
+        for (int x = 0; x < 4; x++) out.val[x] = S.val[x].get();
+        out.coo_idx = S.coo_idx.get();
+        out.csr_rowptr = S.csr_rowptr.get();
+        out.csr_col = S.csr_col.get<uint8_t>();
+        out.bm_words = S.bm_words.get<unsigned long long>();
+        out.coo_idx_bytes = S.coo_idx_bytes;
+        out.csr_rp_bytes = S.csr_rp_bytes;
+        out.val_bytes = vb;
+        out.W = S.W;
+        k_emit<<<grid_for(nnz), kT, 0, st>>>(flag.get<uint32_t>(), excl.get<uint32_t>(), bstart.get<uint32_t>(),
+                                             sch.get<uint8_t>(), perm.get<uint32_t>(), rows.get<uint32_t>(), col, val,
+                                             nnz, S.s, S.k, ord, eoff, out);
+        ABHSF_HIP(hipGetLastError());
+    }
+    ABHSF_HIP(hipStreamSynchronize(st));
+    S.built = true;
+    S.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace abhsf

@@ -1,0 +1,242 @@
+// capi.hip -- the C ABI of libabhsf.so (declared in include/abhsf_c.h).
+#include "storage.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+struct abhsf_handle {
+    abhsf_options opts;
+    hipStream_t stream;
+    abhsf::Storage S;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+abhsf_status fail(abhsf_status code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+abhsf_status guarded(F &&f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return ABHSF_OK;
+    } catch (const abhsf::Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(ABHSF_ERR_ALLOC, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(ABHSF_ERR_INVALID, e.what());
+    }
+}
+
+bool is_pow2_block(int s) { return s >= 2 && s <= 256 && (s & (s - 1)) == 0; }
+
+void reset_storage(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz)
+{
+    h->S.~Storage();
+    new (&h->S) abhsf::Storage();
+    abhsf::Storage &S = h->S;
+    S.set_shape(m, n, h->opts.block_size);
+    S.nnz = nnz;
+    S.policy = h->opts.policy;
+    S.is_binary = h->opts.is_binary;
+    S.val_bytes = h->opts.precision == ABHSF_F32 ? 4 : 8;
+    // value bits of the cost model: test.cpp:581-593 (REAL -> 64, BINARY -> 0),
+    // mmf.cpp:34-47 (single precision -> 32)
+    S.cost_b = h->opts.is_binary ? 0 : (uint64_t)S.val_bytes * 8;
+}
+
+void check_shape(int64_t m, int64_t n, int64_t nnz)
+{
+    ABHSF_REQUIRE(m >= 1 && n >= 1 && m <= INT32_MAX && n <= INT32_MAX, ABHSF_ERR_INVALID,
+                  "matrix dimensions must be in [1, 2^31-1]");
+    ABHSF_REQUIRE(nnz >= 0, ABHSF_ERR_INVALID, "nnz must be non-negative");
+    ABHSF_REQUIRE((uint64_t)nnz < (1ull << 32), ABHSF_ERR_UNSUPPORTED, "nnz must be below 2^32 per matrix");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *abhsf_last_error(void) { return g_last_error.c_str(); }
+
+const char *abhsf_version(void) { return "abhsf-mi355x 0.1 (gfx950)"; }
+
+abhsf_status abhsf_create(const abhsf_options *opts, abhsf_handle **out)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(opts && out, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(is_pow2_block(opts->block_size), ABHSF_ERR_INVALID, "block_size must be a power of two in [2, 256]");
+        ABHSF_REQUIRE(opts->policy == ABHSF_POLICY_PREPROC || opts->policy == ABHSF_POLICY_MSMF, ABHSF_ERR_INVALID,
+                      "unknown policy");
+        ABHSF_REQUIRE(opts->precision == ABHSF_F64 || opts->precision == ABHSF_F32, ABHSF_ERR_INVALID,
+                      "unknown precision");
+        ABHSF_REQUIRE(opts->variant >= ABHSF_VARIANT_AUTO && opts->variant <= ABHSF_VARIANT_BALANCED, ABHSF_ERR_INVALID,
+                      "unknown variant");
+        *out = nullptr;
+        abhsf_handle *h = new abhsf_handle();
+        h->opts = *opts;
+        h->stream = static_cast<hipStream_t>(opts->stream);
+        *out = h;
+    });
+}
+
+abhsf_status abhsf_destroy(abhsf_handle *h)
+{
+    return guarded([&] {
+        if (!h) return;
+        (void)hipStreamSynchronize(h->stream);
+        delete h;
+    });
+}
+
+abhsf_status abhsf_build_csr(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz, const int64_t *rowptr,
+                             const int32_t *col, const double *val, int32_t on_device)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h && rowptr, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(nnz == 0 || col, ABHSF_ERR_INVALID, "null column array");
+        check_shape(m, n, nnz);
+        reset_storage(h, m, n, nnz);
+        if (on_device) {
+            abhsf::build_from_device_csr(h->S, rowptr, col, val, h->stream);
+            return;
+        }
+        ABHSF_REQUIRE(rowptr[0] == 0 && rowptr[m] == nnz, ABHSF_ERR_INVALID, "row pointers do not span nnz");
+        abhsf::DevBuf drp(((uint64_t)m + 1) * 8), dc((uint64_t)nnz * 4), dv(val ? (uint64_t)nnz * 8 : 0);
+        ABHSF_HIP(hipMemcpyAsync(drp.get(), rowptr, drp.bytes(), hipMemcpyHostToDevice, h->stream));
+        if (nnz) ABHSF_HIP(hipMemcpyAsync(dc.get(), col, dc.bytes(), hipMemcpyHostToDevice, h->stream));
+        if (val && nnz) ABHSF_HIP(hipMemcpyAsync(dv.get(), val, dv.bytes(), hipMemcpyHostToDevice, h->stream));
+        abhsf::build_from_device_csr(h->S, drp.get<int64_t>(), dc.get<int32_t>(), val ? dv.get<double>() : nullptr,
+                                     h->stream);
+    });
+}
+
+abhsf_status abhsf_build_coo(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz, const int32_t *rows,
+                             const int32_t *cols, const double *vals)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h, ABHSF_ERR_INVALID, "null handle");
+        ABHSF_REQUIRE(nnz == 0 || (rows && cols), ABHSF_ERR_INVALID, "null triplet array");
+        check_shape(m, n, nnz);
+        for (int64_t i = 0; i < nnz; i++)
+            ABHSF_REQUIRE(rows[i] >= 0 && rows[i] < m && cols[i] >= 0 && cols[i] < n, ABHSF_ERR_INVALID,
+                          "triplet index out of range");
+        // private copy sorted row-major (the reference sorts the caller's
+        // vector in place, spmv/unified/spmv.cpp:77)
+        std::vector<uint64_t> order((size_t)nnz);
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+            return rows[a] != rows[b] ? rows[a] < rows[b] : cols[a] < cols[b];
+        });
+        std::vector<int64_t> rp((size_t)m + 1, 0);
+        std::vector<int32_t> c((size_t)nnz);
+        std::vector<double> v(vals ? (size_t)nnz : 0);
+        for (int64_t i = 0; i < nnz; i++) {
+            const uint64_t o = order[(size_t)i];
+            if (i > 0) {
+                const uint64_t p = order[(size_t)i - 1];
+                ABHSF_REQUIRE(!(rows[o] == rows[p] && cols[o] == cols[p]), ABHSF_ERR_INVALID, "duplicate triplet");
+            }
+            rp[(size_t)rows[o] + 1]++;
+            c[(size_t)i] = cols[o];
+            if (vals) v[(size_t)i] = vals[o];
+        }
+        for (int64_t r = 0; r < m; r++) rp[(size_t)r + 1] += rp[(size_t)r];
+        reset_storage(h, m, n, nnz);
+        abhsf::DevBuf drp(((uint64_t)m + 1) * 8), dc((uint64_t)nnz * 4), dv(vals ? (uint64_t)nnz * 8 : 0);
+        ABHSF_HIP(hipMemcpyAsync(drp.get(), rp.data(), drp.bytes(), hipMemcpyHostToDevice, h->stream));
+        if (nnz) ABHSF_HIP(hipMemcpyAsync(dc.get(), c.data(), dc.bytes(), hipMemcpyHostToDevice, h->stream));
+        if (vals && nnz) ABHSF_HIP(hipMemcpyAsync(dv.get(), v.data(), dv.bytes(), hipMemcpyHostToDevice, h->stream));
+        abhsf::build_from_device_csr(h->S, drp.get<int64_t>(), dc.get<int32_t>(), vals ? dv.get<double>() : nullptr,
+                                     h->stream);
+    });
+}
+
+abhsf_status abhsf_get_stats(const abhsf_handle *h, abhsf_stats *out)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h && out, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(h->S.built, ABHSF_ERR_STATE, "matrix not built");
+        const abhsf::Storage &S = h->S;
+        std::memset(out, 0, sizeof(*out));
+        out->m = S.m; out->n = S.n; out->nnz = S.nnz;
+        out->s = S.s; out->k = S.k; out->M = S.M; out->N = S.N;
+        for (int x = 0; x < 4; x++) { out->blocks[x] = S.nb[x]; out->values[x] = S.ne[x]; }
+        out->smf_bits_preproc = S.smf_preproc;
+        out->smf_bits_msmf = S.smf_msmf;
+        out->stored_bytes = S.stored_bytes();
+        out->index_bytes = S.stored_bytes() - S.value_bytes();
+        out->build_ms = S.build_ms;
+    });
+}
+
+abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const double *X, int64_t ldx, double beta,
+                        double *Y, int64_t ldy)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h && X && Y, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(h->S.built, ABHSF_ERR_STATE, "matrix not built");
+        ABHSF_REQUIRE(nrhs >= 1 && nrhs <= 64, ABHSF_ERR_UNSUPPORTED, "nrhs must be in [1, 64]");
+        ABHSF_REQUIRE(ldx >= nrhs && ldy >= nrhs, ABHSF_ERR_INVALID, "leading dimension smaller than nrhs");
+        abhsf::spmm(h->S.launch(), h->S.view(), nrhs, alpha, X, ldx, beta, Y, ldy, h->stream);
+    });
+}
+
+abhsf_status abhsf_spmv(abhsf_handle *h, double alpha, const double *x, double beta, double *y)
+{
+    return abhsf_spmm(h, 1, alpha, x, 1, beta, y, 1);
+}
+
+abhsf_status abhsf_export(const abhsf_handle *h, const char *name, void *dst, size_t cap, size_t *bytes)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h && name, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(h->S.built, ABHSF_ERR_STATE, "matrix not built");
+        const abhsf::Storage &S = h->S;
+        static const char *pre[4] = {"coo_", "csr_", "bm_", "dn_"};
+        const abhsf::DevBuf *buf = nullptr;
+        const std::string nm(name);
+        for (int x = 0; x < 4 && !buf; x++) {
+            if (nm == std::string(pre[x]) + "sptr") buf = &S.sptr[x];
+            else if (nm == std::string(pre[x]) + "eptr") buf = &S.eptr[x];
+            else if (nm == std::string(pre[x]) + "bcol") buf = &S.bcol[x];
+            else if (nm == std::string(pre[x]) + "z1") buf = &S.z1[x];
+            else if (nm == std::string(pre[x]) + "val") buf = &S.val[x];
+        }
+        if (!buf) {
+            if (nm == "coo_idx") buf = &S.coo_idx;
+            else if (nm == "csr_rowptr") buf = &S.csr_rowptr;
+            else if (nm == "csr_col") buf = &S.csr_col;
+            else if (nm == "bm_words") buf = &S.bm_words;
+        }
+        ABHSF_REQUIRE(buf, ABHSF_ERR_INVALID, "unknown array name: " + nm);
+        if (bytes) *bytes = buf->bytes();
+        if (!dst) return;
+        ABHSF_REQUIRE(cap >= buf->bytes(), ABHSF_ERR_INVALID, "destination too small");
+        if (buf->bytes()) {
+            ABHSF_HIP(hipMemcpyAsync(dst, buf->get(), buf->bytes(), hipMemcpyDeviceToHost, h->stream));
+            ABHSF_HIP(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
+abhsf_status abhsf_synchronize(abhsf_handle *h)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h, ABHSF_ERR_INVALID, "null handle");
+        ABHSF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// storage.hpp -- the ABHSF storage owned by a handle, and its GPU builder.
+#pragma once
+
+#include "prims.hpp"
+#include "spmv.hpp"
+
+namespace abhsf {
+
+struct Storage {
+    // shape and options
+    int64_t m = 0, n = 0, nnz = 0, M = 0, N = 0;
+    int s = 0, k = 0, W = 0;
+    int policy = ABHSF_POLICY_PREPROC, is_binary = 0, val_bytes = 8;
+    uint64_t cost_b = 64;
+    int bcol_bytes = 4, coo_idx_bytes = 2, csr_rp_bytes = 2;
+    bool built = false;
+    // per scheme
+    uint64_t nb[4] = {0, 0, 0, 0}, ne[4] = {0, 0, 0, 0};
+    DevBuf sptr[4], eptr[4], bcol[4], z1[4], val[4];
+    DevBuf coo_idx, csr_rowptr, csr_col, bm_words;
+    // footprints (bits) under both cost models
+    uint64_t smf_preproc = 0, smf_msmf = 0;
+    double build_ms = 0.0;
+
+    DevMat view() const;
+    SpmvLaunch launch() const { return SpmvLaunch{bcol_bytes, val_bytes}; }
+    uint64_t stored_bytes() const;
+    uint64_t value_bytes() const;
+    void set_shape(int64_t m_, int64_t n_, int s_);
+};
+
+// Build S from a device CSR (rowptr int64[m+1], col int32[nnz], val f64[nnz]).
+// S's shape/options must be set; throws abhsf::Error on invalid input.
+void build_from_device_csr(Storage &S, const int64_t *rowptr, const int32_t *col, const double *val,
+                           hipStream_t st);
+
+}  // namespace abhsf

@@ -1,0 +1,164 @@
+"""Host-side mirror of the reference's matrix-class interface for this path.
+
+The reference's kernel interface (spmv/unified/spmv.cpp:71-242) is a class with
+  from_elements(elements, props)  -- build from (row, col, val) triplets
+  spmv(x, y)                      -- y += A*x (accumulating), synchronous
+  release()                       -- free storage
+AbhsfMatrix keeps those names and meanings on top of the C ABI, plus the
+extensions of this library (alpha/beta SpMV, multi-RHS SpMM, statistics and
+array export).  x and y are device arrays (hip.DeviceArray or raw addresses),
+as for the reference's GPU variant spmv_cusparse_hyb (spmv/unified/spmv.cpp:199-207).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi, hip
+
+
+class matrix_type_t(enum.IntEnum):  # include/abhsf/utils/matrix_properties.h:8-14
+    BINARY = 0
+    INTEGER = 1
+    REAL = 2
+    COMPLEX = 3
+
+
+class matrix_symmetry_t(enum.IntEnum):  # include/abhsf/utils/matrix_properties.h:16-22
+    SYMMETRIC = 0
+    HERMITIAN = 1
+    SKEW_SYMMETRIC = 2
+    UNSYMMETRIC = 3
+
+
+@dataclass
+class matrix_properties:  # include/abhsf/utils/matrix_properties.h:24-71
+    m: int = 0
+    n: int = 0
+    nnz: int = 0
+    type: matrix_type_t = matrix_type_t.REAL
+    symmetry: matrix_symmetry_t = matrix_symmetry_t.UNSYMMETRIC
+
+
+def _addr(a) -> int:
+    if isinstance(a, hip.DeviceArray):
+        return a.address
+    return int(a)
+
+
+_POLICY = {"preproc": capi.POLICY_PREPROC, "msmf": capi.POLICY_MSMF}
+_PREC = {"f64": capi.F64, "f32": capi.F32}
+_VAR = {"auto": capi.VARIANT_AUTO, "strip": capi.VARIANT_STRIP, "balanced": capi.VARIANT_BALANCED}
+
+
+class AbhsfMatrix:
+    """One ABHSF matrix resident in HBM."""
+
+    def __init__(self, block_size: int = 64, policy: str = "preproc", precision: str = "f64",
+                 variant: str = "auto", stream=None, is_binary: bool = False):
+        self.opts = capi.Options(block_size, _POLICY[policy], _PREC[precision], _VAR[variant],
+                                 int(is_binary), 0,
+                                 stream.handle if isinstance(stream, hip.Stream) else stream)
+        self._h = C.c_void_p(0)
+        capi.call("abhsf_create", C.byref(self.opts), C.byref(self._h))
+        self.props = None
+
+    # -- build ---------------------------------------------------------------
+    def from_elements(self, elements, props: matrix_properties) -> None:
+        """Build from triplets (rows, cols, vals) arrays or a sequence of
+        (row, col, val) tuples.  Unlike spmv/unified/spmv.cpp:77 the caller's
+        data is not reordered (a private copy is sorted)."""
+        if isinstance(elements, tuple) and len(elements) == 3 and hasattr(elements[0], "__len__"):
+            rows, cols, vals = elements
+        else:
+            arr = np.asarray(elements, dtype=np.float64).reshape(-1, 3)
+            rows, cols, vals = arr[:, 0], arr[:, 1], arr[:, 2]
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = None if vals is None else np.ascontiguousarray(vals, dtype=np.float64)
+        capi.call("abhsf_build_coo", self._h, props.m, props.n, rows.size,
+                  rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                  None if vals is None else vals.ctypes.data_as(C.c_void_p))
+        self.props = props
+
+    def from_csr(self, m: int, n: int, rowptr, col, val, on_device: bool = False, nnz: int = None) -> None:
+        """Build from CSR (host numpy arrays, or device arrays with on_device)."""
+        if on_device:
+            if nnz is None:
+                nnz = col.shape[0]
+            capi.call("abhsf_build_csr", self._h, m, n, nnz, _addr(rowptr), _addr(col),
+                      None if val is None else _addr(val), 1)
+        else:
+            rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+            col = np.ascontiguousarray(col, dtype=np.int32)
+            val = None if val is None else np.ascontiguousarray(val, dtype=np.float64)
+            capi.call("abhsf_build_csr", self._h, m, n, int(rowptr[-1]),
+                      rowptr.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p),
+                      None if val is None else val.ctypes.data_as(C.c_void_p), 0)
+        self.props = matrix_properties(m, n, self.stats()["nnz"])
+
+    # -- apply ---------------------------------------------------------------
+    def spmv(self, x, y) -> None:
+        """y += A*x and wait (spmv/unified/spmv.cpp:140-159, 199-207)."""
+        capi.call("abhsf_spmv", self._h, 1.0, _addr(x), 1.0, _addr(y))
+        capi.call("abhsf_synchronize", self._h)
+
+    def spmv_async(self, alpha: float, x, beta: float, y) -> None:
+        """y = alpha*A*x + beta*y, enqueued on the handle's stream."""
+        capi.call("abhsf_spmv", self._h, float(alpha), _addr(x), float(beta), _addr(y))
+
+    def spmm_async(self, nrhs: int, alpha: float, X, ldx: int, beta: float, Y, ldy: int) -> None:
+        capi.call("abhsf_spmm", self._h, int(nrhs), float(alpha), _addr(X), int(ldx), float(beta),
+                  _addr(Y), int(ldy))
+
+    def synchronize(self) -> None:
+        capi.call("abhsf_synchronize", self._h)
+
+    # -- inspection ------------------------------------------------------------
+    def stats(self) -> dict:
+        st = capi.Stats()
+        capi.call("abhsf_get_stats", self._h, C.byref(st))
+        return st.as_dict()
+
+    def export(self, name: str, dtype) -> np.ndarray:
+        nbytes = C.c_size_t(0)
+        capi.call("abhsf_export", self._h, name.encode(), None, 0, C.byref(nbytes))
+        out = np.empty(nbytes.value // np.dtype(dtype).itemsize, dtype=dtype)
+        if nbytes.value:
+            capi.call("abhsf_export", self._h, name.encode(), out.ctypes.data_as(C.c_void_p),
+                      nbytes.value, C.byref(nbytes))
+        return out
+
+    def export_all(self) -> dict:
+        """All storage arrays with their stored element types."""
+        st = self.stats()
+        s, N = st["s"], st["N"]
+        bdt = np.uint16 if N <= 65536 else np.uint32
+        vdt = np.float64 if self.opts.precision == capi.F64 else np.float32
+        out = {}
+        for p in ("coo", "csr", "bm", "dn"):
+            out[f"{p}_sptr"] = self.export(f"{p}_sptr", np.uint32)
+            out[f"{p}_eptr"] = self.export(f"{p}_eptr", np.uint32)
+            out[f"{p}_bcol"] = self.export(f"{p}_bcol", bdt)
+            out[f"{p}_z1"] = self.export(f"{p}_z1", np.uint16)
+            out[f"{p}_val"] = self.export(f"{p}_val", vdt)
+        out["coo_idx"] = self.export("coo_idx", np.uint8 if s <= 16 else np.uint16)
+        out["csr_rowptr"] = self.export("csr_rowptr", np.uint8 if s <= 8 else np.uint16)
+        out["csr_col"] = self.export("csr_col", np.uint8)
+        out["bm_words"] = self.export("bm_words", np.uint64)
+        return out
+
+    # -- release ---------------------------------------------------------------
+    def release(self) -> None:
+        if self._h and self._h.value:
+            capi.call("abhsf_destroy", self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

@@ -1,0 +1,158 @@
+/*
+ * abhsf_c.h -- C ABI of the MI355X ABHSF SpMV library (libabhsf.so).
+ *
+ * Adaptive-blocking hierarchical storage format: the matrix is tiled into
+ * s x s blocks (s = 2^k, 2 <= s <= 256); every nonzero block is stored in the
+ * scheme (COO, CSR, bitmap or dense) that the cost model makes smallest, and the
+ * nonzero blocks of each scheme are indexed by a block-level CSR over block
+ * rows ("strips").
+ *
+ * Conventions
+ *   - Every call returns abhsf_status; 0 is success.  abhsf_last_error()
+ *     gives the message of the last failure on the calling thread.
+ *   - The handle owns all device memory it allocates.  One handle must not
+ *     be used from two threads at once.
+ *   - x / y / X / Y are DEVICE pointers (hipMalloc'd).  Compute calls are
+ *     enqueued on the handle's stream and return without synchronising;
+ *     abhsf_synchronize() waits.
+ *   - The caller's input arrays are never modified (the reference's
+ *     csr_matrix::from_elements sorts the caller's vector in place,
+ *     spmv/unified/spmv.cpp:77; this library sorts a private copy).
+ *
+ * Storage layout in HBM (one set of arrays per scheme X in {coo,csr,bm,dn}):
+ *   X_sptr[M+1]  u32   block offsets per strip (block-level CSR row pointers)
+ *   X_eptr[M+1]  u32   value offsets per strip
+ *   X_bcol[nb]   u16 if N <= 65536 else u32   block column J
+ *   X_z1[nb]     u16   entries in the block minus one
+ *   X_val[ne]    f64 or f32 (ABHSF_F32)
+ *   coo_idx[ne]  (lrow << k) | lcol, u8 if s <= 16 else u16
+ *   csr_rowptr[nb*(s+1)] u8 if s <= 8 else u16; csr_col[ne] u8 = lcol
+ *   bm_words[nb*max(1,s*s/64)] u64, bit lrow*s+lcol LSB-first; values in
+ *                set-bit order
+ *   dn_val[nb*s*s] row-major, explicit zeros
+ * Blocks of a strip are in block-column order inside each scheme group;
+ * entries of a block are in (lrow, lcol) order.
+ */
+#ifndef ABHSF_C_H
+#define ABHSF_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t abhsf_status;
+enum {
+    ABHSF_OK = 0,
+    ABHSF_ERR_INVALID = 1,     /* bad argument or malformed matrix */
+    ABHSF_ERR_HIP = 2,         /* HIP runtime failure */
+    ABHSF_ERR_ALLOC = 3,       /* device or host allocation failed */
+    ABHSF_ERR_STATE = 4,       /* call out of order (e.g. spmv before build) */
+    ABHSF_ERR_UNSUPPORTED = 5  /* size or option outside what is implemented */
+};
+
+/* Scheme-choice cost models.
+ * PREPROC: preprocessing/test.cpp:55-77 (abhsf_structure_memory_footprint_processor)
+ * MSMF:    include/abhsf/msmf.h:9-48 (block_min_msmf) */
+enum { ABHSF_POLICY_PREPROC = 0, ABHSF_POLICY_MSMF = 1 };
+
+/* Value storage.  F64 is the reference-equivalent path (real_type = double,
+ * spmv/unified/spmv.cpp:59).  F32 stores values in single precision and
+ * accumulates in double with compensated (TwoSum) sums. */
+enum { ABHSF_F64 = 0, ABHSF_F32 = 1 };
+
+/* SpMV kernel variants (the tuner chooses among them). */
+enum {
+    ABHSF_VARIANT_AUTO = 0,
+    ABHSF_VARIANT_STRIP = 1,    /* one wavefront per strip */
+    ABHSF_VARIANT_BALANCED = 2  /* heavy strips split over several wavefronts */
+};
+
+typedef struct abhsf_options {
+    int32_t block_size;   /* s: power of two in [2, 256] */
+    int32_t policy;       /* ABHSF_POLICY_* */
+    int32_t precision;    /* ABHSF_F64 / ABHSF_F32 */
+    int32_t variant;      /* ABHSF_VARIANT_* */
+    int32_t is_binary;    /* pattern matrix: dense scheme excluded (test.cpp:70) */
+    int32_t reserved;
+    void *stream;         /* hipStream_t; NULL = null stream */
+} abhsf_options;
+
+typedef struct abhsf_stats {
+    int64_t m, n, nnz;
+    int32_t s, k;
+    int64_t M, N;                  /* block rows / block columns */
+    uint64_t blocks[4];            /* nonzero blocks per scheme: COO, CSR, bitmap, dense */
+    uint64_t values[4];            /* stored values per scheme (dense includes zeros) */
+    uint64_t smf_bits_preproc;     /* structure bits, preprocessing/test.cpp:76-85 */
+    uint64_t smf_bits_msmf;        /* structure bits, block_statistics/mmf.cpp:68-101 (adaptive, no values) */
+    uint64_t stored_bytes;         /* bytes of all storage arrays as stored in HBM */
+    uint64_t index_bytes;          /* stored_bytes minus value bytes */
+    double build_ms;               /* device time of the last build */
+} abhsf_stats;
+
+typedef struct abhsf_handle abhsf_handle;
+
+/* Create an empty matrix handle.  Replaces the construction of the
+ * reference's matrix classes (spmv/unified/spmv.cpp:71, 1205). */
+abhsf_status abhsf_create(const abhsf_options *opts, abhsf_handle **out);
+
+/* Build the ABHSF storage from a CSR matrix with 0-based indices, columns
+ * sorted ascending and unique within each row.  on_device = 1: the three
+ * arrays are device pointers; 0: host pointers (copied to the device).
+ * The conversion runs on the GPU (key, sort, head-flag scan, per-block cost
+ * and choice, per-scheme compaction, emit).
+ * Replaces csr_matrix::from_elements (spmv/unified/spmv.cpp:74-104) and the
+ * block iteration + footprint processor of preprocessing/test.cpp:36-108,237-368. */
+abhsf_status abhsf_build_csr(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
+                             const int64_t *rowptr, const int32_t *col, const double *val,
+                             int32_t on_device);
+
+/* Build from host COO triplets (any order, no duplicates); a private copy is
+ * sorted row-major first.  Mirrors element_t = (row, col, val)
+ * (spmv/unified/spmv.cpp:68-69) as three arrays. */
+abhsf_status abhsf_build_coo(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
+                             const int32_t *rows, const int32_t *cols, const double *vals);
+
+/* Storage statistics (blocks per scheme, footprints under both cost models). */
+abhsf_status abhsf_get_stats(const abhsf_handle *h, abhsf_stats *out);
+
+/* y = alpha * A * x + beta * y (device pointers; beta == 0 does not read y).
+ * With alpha = beta = 1 this is the reference's accumulating
+ * csr_matrix::spmv(x, y) (spmv/unified/spmv.cpp:140-159) and
+ * spmv_cusparse_hyb (spmv/unified/spmv.cpp:199-207, which also syncs). */
+abhsf_status abhsf_spmv(abhsf_handle *h, double alpha, const double *x, double beta, double *y);
+
+/* Y = alpha * A * X + beta * Y for nrhs right-hand sides, X (n x nrhs) and
+ * Y (m x nrhs) row-major with leading dimensions ldx, ldy >= nrhs.
+ * nrhs in [1, 64].  No reference counterpart (batched multi-RHS extension). */
+abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const double *X, int64_t ldx,
+                        double beta, double *Y, int64_t ldy);
+
+/* Copy one storage array to host memory.  name is one of the array names in
+ * the layout above ("coo_sptr", "coo_eptr", "coo_bcol", "coo_z1", "coo_val",
+ * "coo_idx", "csr_...", "csr_rowptr", "csr_col", "bm_...", "bm_words",
+ * "dn_..."), *bytes receives the array size; dst may be NULL to query it.
+ * Used to check the GPU conversion byte-for-byte. */
+abhsf_status abhsf_export(const abhsf_handle *h, const char *name, void *dst, size_t cap,
+                          size_t *bytes);
+
+/* Wait for all work enqueued on the handle's stream. */
+abhsf_status abhsf_synchronize(abhsf_handle *h);
+
+/* Release device memory and the handle.  Replaces csr_matrix::release
+ * (spmv/unified/spmv.cpp:210-226). */
+abhsf_status abhsf_destroy(abhsf_handle *h);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *abhsf_last_error(void);
+
+/* Library version string. */
+const char *abhsf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

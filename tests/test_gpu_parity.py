@@ -1,0 +1,253 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Conversion: every storage array must equal the oracle builder's byte for byte.
+SpMV/SpMM FP64: relative max-norm error <= 1e-12 against the textbook CSR FP64
+SpMV of the same inputs (TOL below).  FP32 value storage: error against the
+FP64 oracle on the FP32-rounded values <= 1e-12 (compensated accumulation),
+and the error against the FP64 values is reported.
+"""
+import numpy as np
+import pytest
+
+from abhsf_amd import AbhsfMatrix, AbhsfError, capi, synth
+from abhsf_amd.hip import DeviceArray
+from oracle import pyoracle
+
+from conftest import ROOT, read_mtx
+
+TOL = 1e-12  # FP64 relative max-norm bound
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(got, ref):
+    den = np.max(np.abs(ref))
+    return float(np.max(np.abs(got - ref)) / (den if den > 0 else 1.0))
+
+
+def build(A, s, policy="preproc", precision="f64"):
+    M = AbhsfMatrix(block_size=s, policy=policy, precision=precision)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    return M
+
+
+def check_arrays(M, A, s, policy=pyoracle.PREPROC, val_bytes=8):
+    k = s.bit_length() - 1
+    O = pyoracle.Abhsf(A.m, A.n, A.rowptr, A.col, A.val, k, policy, val_bytes)
+    oa, ga = O.arrays(), M.export_all()
+    for name in oa:
+        assert oa[name].dtype == ga[name].dtype, name
+        assert np.array_equal(oa[name], ga[name]), f"array {name} differs (s={s})"
+    st = M.stats()
+    assert [st["blocks"][x] for x in capi.SCHEMES] == O.nb
+    return O, st
+
+
+def spmv_check(M, A, x, y0=None, alpha=1.0, beta=0.0):
+    dx = DeviceArray.from_numpy(x)
+    dy = DeviceArray.from_numpy(y0 if y0 is not None else np.zeros(A.m))
+    M.spmv_async(alpha, dx, beta, dy)
+    M.synchronize()
+    got = dy.numpy()
+    ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x, alpha, beta, y0)
+    return got, ref
+
+
+MATS = {
+    "laplacian_32": lambda: synth.laplacian2d(32),
+    "random_300x257": lambda: synth.random_sparse(300, 257, 0.05, seed=3),
+    "random_dense_ish": lambda: synth.random_sparse(130, 140, 0.6, seed=4),
+    "block_mix_4096": lambda: synth.block_mix(4096, seed=1),
+    "rmat_12": lambda: synth.rmat(12, 8, seed=2),
+    "banded_1000": lambda: synth.banded(1000, 9, seed=5),
+    "random_blocks_2000": lambda: synth.random_blocks(2000, 16, 4, 0.3, seed=6),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+@pytest.mark.parametrize("s", [2, 4, 8, 16, 32, 64, 128, 256])
+def test_conversion_bit_exact_and_spmv(gpu, name, s):
+    A = MATS[name]()
+    print(f"case {name} s={s} m={A.m} n={A.n} nnz={A.nnz}", flush=True)
+    M = build(A, s)
+    check_arrays(M, A, s)
+    x = np.random.default_rng(s).uniform(-1, 1, A.n)
+    y0 = np.random.default_rng(s + 1).uniform(-1, 1, A.m)
+    got, ref = spmv_check(M, A, x, y0, alpha=1.0, beta=1.0)
+    assert relerr(got, ref) <= TOL
+
+
+def test_config1_laplacian256_s64(gpu):
+    """256^2 5-point Laplacian, s=64, x ~ U[-1,1] seed 0,
+    y = A x checked against CSR FP64; footprint = the reference's 3,695,344 bits."""
+    A = synth.laplacian2d(256)
+    assert A.nnz == 326656
+    M = build(A, 64)
+    O, st = check_arrays(M, A, 64)
+    assert st["smf_bits_preproc"] == 3695344
+    assert st["blocks"]["csr"] == 1024 and st["blocks"]["coo"] == 3576
+    x = np.random.default_rng(0).uniform(-1, 1, A.n)
+    got, ref = spmv_check(M, A, x)
+    assert relerr(got, ref) <= TOL
+    # closed form: A * 1 = 4 - (number of neighbours)
+    ones = np.ones(A.n)
+    got1, _ = spmv_check(M, A, ones)
+    g = 256
+    i, j = np.divmod(np.arange(A.n), g)
+    deg = (i > 0).astype(int) + (i < g - 1) + (j > 0) + (j < g - 1)
+    assert np.array_equal(got1, 4.0 - deg)
+
+
+def test_test_mtx_all_block_sizes(gpu):
+    m, n, r, c, v = read_mtx(f"{ROOT}/tests/golden/test.mtx")
+    A = synth.from_triplets(m, n, r, c, v)
+    import json
+    ka = json.load(open(f"{ROOT}/tests/golden/known_answers.json"))["test_mtx"]
+    for s in (2, 4, 8, 16, 32, 64, 128, 256):
+        M = build(A, s)
+        check_arrays(M, A, s)
+        assert M.stats()["smf_bits_preproc"] == ka["preproc_smf_bits"][str(s)]
+        x = np.arange(1, n + 1, dtype=np.float64)
+        got, ref = spmv_check(M, A, x)
+        assert relerr(got, ref) <= TOL
+
+
+def test_from_elements_accumulating_harness(gpu):
+    """Reference harness semantics (spmv/unified/spmv.cpp:1176-1232): x = 0.5,
+    y = 0, 5 warm-up + 20 timed accumulating spmv calls; result ||y||/25 equals
+    the expected ||A*0.5||."""
+    A = synth.laplacian2d(40)
+    r, c, v = A.triplets()
+    from abhsf_amd import matrix_properties
+    perm = np.random.default_rng(9).permutation(r.size)  # any order is accepted
+    M = AbhsfMatrix(block_size=8)
+    M.from_elements((r[perm], c[perm], v[perm]), matrix_properties(A.m, A.n, A.nnz))
+    check_arrays(M, A, 8)
+    x = DeviceArray.from_numpy(np.full(A.n, 0.5))
+    y = DeviceArray.zeros(A.m)
+    for _ in range(25):
+        M.spmv(x, y)
+    expected = np.linalg.norm(pyoracle.csr_spmv(A.rowptr, A.col, A.val, np.full(A.n, 0.5)))
+    assert abs(np.linalg.norm(y.numpy()) / 25 - expected) <= 1e-12 * expected
+
+
+@pytest.mark.parametrize("s", [8, 32, 64, 256])
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 4, 8, 16, 32, 33, 64])
+def test_spmm(gpu, s, nrhs):
+    A = synth.block_mix(2048, seed=7)
+    print(f"case spmm s={s} nrhs={nrhs}", flush=True)
+    M = build(A, s)
+    rng = np.random.default_rng(nrhs)
+    ldx, ldy = nrhs + 3, nrhs + 1
+    X = rng.uniform(-1, 1, (A.n, ldx))
+    Y0 = rng.uniform(-1, 1, (A.m, ldy))
+    dX, dY = DeviceArray.from_numpy(X), DeviceArray.from_numpy(Y0)
+    M.spmm_async(nrhs, 0.5, dX, ldx, -2.0, dY, ldy)
+    M.synchronize()
+    got = dY.numpy()
+    for j in range(nrhs):
+        ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, X[:, j].copy(), 0.5, -2.0, Y0[:, j].copy())
+        assert relerr(got[:, j], ref) <= TOL, j
+    assert np.array_equal(got[:, nrhs:], Y0[:, nrhs:])  # padding columns untouched
+
+
+@pytest.mark.parametrize("s", [8, 32, 64])
+def test_msmf_policy(gpu, s):
+    A = synth.block_mix(4096, seed=11)
+    M = build(A, s, policy="msmf")
+    check_arrays(M, A, s, policy=pyoracle.MSMF)
+    x = np.random.default_rng(1).uniform(-1, 1, A.n)
+    got, ref = spmv_check(M, A, x)
+    assert relerr(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("s", [8, 32, 64, 128])
+def test_fp32_storage_compensated(gpu, s):
+    A = synth.block_mix(4096, seed=12)
+    M = build(A, s, precision="f32")
+    check_arrays(M, A, s, val_bytes=4)
+    x = np.random.default_rng(2).uniform(-1, 1, A.n)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.zeros(A.m)
+    M.spmv_async(1.0, dx, 0.0, dy)
+    M.synchronize()
+    got = dy.numpy()
+    v32 = A.val.astype(np.float32).astype(np.float64)
+    ref32 = pyoracle.csr_spmv(A.rowptr, A.col, v32, x)
+    ref64 = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)
+    e32, e64 = relerr(got, ref32), relerr(got, ref64)
+    print(f"fp32 storage s={s}: err vs fp64 oracle on f32 values {e32:.2e}, vs fp64 values {e64:.2e}")
+    assert e32 <= TOL
+    assert e64 <= 1e-6
+
+
+def test_beta_zero_ignores_y_and_alpha(gpu):
+    A = synth.random_sparse(500, 400, 0.02, seed=8)
+    M = build(A, 16)
+    x = np.random.default_rng(3).uniform(-1, 1, A.n)
+    y0 = np.full(A.m, np.nan)
+    got, ref = spmv_check(M, A, x, y0, alpha=-3.0, beta=0.0)
+    assert not np.any(np.isnan(got))
+    assert relerr(got, ref) <= TOL
+
+
+def full_matrix(m, n):
+    r, c = np.divmod(np.arange(m * n), n)
+    v = np.random.default_rng(0).standard_normal(m * n)
+    return synth.from_triplets(m, n, r, c, v)
+
+
+def test_edge_shapes(gpu):
+    cases = [
+        synth.from_triplets(1, 1, [0], [0], [2.5]),
+        synth.from_triplets(5, 3, [], [], []),                          # empty
+        synth.from_triplets(3, 1000, [0, 2, 2], [999, 0, 998], [1.0, 2.0, 3.0]),
+        synth.from_triplets(1000, 3, [999, 0, 500], [2, 0, 1], [1.0, -1.0, 4.0]),
+        full_matrix(257, 255),                                          # dense blocks at every s
+    ]
+    for A in cases:
+        for s in (2, 8, 64, 256):
+            M = build(A, s)
+            check_arrays(M, A, s)
+            x = np.random.default_rng(0).uniform(-1, 1, A.n)
+            got, ref = spmv_check(M, A, x)
+            assert relerr(got, ref) <= TOL, (A.m, A.n, s)
+
+
+def test_invalid_input_rejected(gpu):
+    M = AbhsfMatrix(block_size=8)
+    # unsorted columns within a row
+    with pytest.raises(AbhsfError) as e:
+        M.from_csr(2, 4, np.array([0, 2, 3]), np.array([3, 1, 0]), np.ones(3))
+    assert e.value.status == capi.ERR_INVALID
+    # column out of range
+    with pytest.raises(AbhsfError):
+        M.from_csr(2, 4, np.array([0, 1, 2]), np.array([0, 4]), np.ones(2))
+    # duplicate triplet
+    from abhsf_amd import matrix_properties
+    with pytest.raises(AbhsfError):
+        M.from_elements((np.array([0, 0]), np.array([1, 1]), np.ones(2)), matrix_properties(2, 2, 2))
+    # spmv before a successful build
+    dx, dy = DeviceArray.zeros(4), DeviceArray.zeros(2)
+    with pytest.raises(AbhsfError) as e:
+        M.spmv(dx, dy)
+    assert e.value.status == capi.ERR_STATE
+
+
+def test_device_csr_input_and_determinism(gpu):
+    A = synth.rmat(14, 16, seed=3)
+    d_rp, d_c, d_v = (DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
+    M = AbhsfMatrix(block_size=32)
+    M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True)
+    check_arrays(M, A, 32)
+    x = np.random.default_rng(5).uniform(-1, 1, A.n)
+    dx = DeviceArray.from_numpy(x)
+    outs = []
+    for _ in range(3):
+        dy = DeviceArray.zeros(A.m)
+        M.spmv_async(1.0, dx, 0.0, dy)
+        M.synchronize()
+        outs.append(dy.numpy())
+    ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)
+    assert relerr(outs[0], ref) <= TOL
+    same = all(np.array_equal(outs[0], o) for o in outs[1:])
+    print(f"bitwise-identical repeated SpMV: {same}")
