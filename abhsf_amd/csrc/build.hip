@@ -98,13 +98,16 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-// z, scheme, per-scheme indicator/value counts, footprint sums.
-__global__ void k_block_scheme(const uint32_t *__restrict__ bstart, uint32_t nblk, int s, int policy,
-                               uint64_t cost_b, int is_binary, uint64_t hdr_bits, uint8_t *__restrict__ sch,
-                               Four<uint32_t *> cnt, Four<uint32_t *> ecnt, unsigned long long *totals)
+// z, scheme, per-scheme indicator/value counts; per-workgroup partial sums of
+// the footprints and value counts (partial[q * gridDim.x + workgroup]).
+__global__ __launch_bounds__(kT) void k_block_scheme(const uint32_t *__restrict__ bstart, uint32_t nblk, int s,
+                                                     int policy, uint64_t cost_b, int is_binary, uint64_t hdr_bits,
+                                                     uint8_t *__restrict__ sch, Four<uint32_t *> cnt,
+                                                     Four<uint32_t *> ecnt, unsigned long long *__restrict__ partial)
 {
+    __shared__ unsigned long long red[kT / kWave][6];
     const uint32_t b = blockIdx.x * kT + threadIdx.x;
-    unsigned long long pre = 0, msmf = 0, ev[4] = {0, 0, 0, 0};
+    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
     if (b < nblk) {
         const uint64_t z = bstart[b + 1] - bstart[b];
         uint64_t bits;
@@ -115,22 +118,23 @@ __global__ void k_block_scheme(const uint32_t *__restrict__ bstart, uint32_t nbl
         for (int q = 0; q < 4; q++) {
             cnt.p[q][b] = (q == x) ? 1u : 0u;
             ecnt.p[q][b] = (q == x) ? evx : 0u;
-            ev[q] = (q == x) ? evx : 0u;
+            v[2 + q] = (q == x) ? evx : 0u;
         }
         choose_scheme(ABHSF_POLICY_PREPROC, (uint64_t)s, z, cost_b, is_binary, &bits);
-        pre = hdr_bits + bits;  // preprocessing/test.cpp:76
+        v[0] = hdr_bits + bits;  // preprocessing/test.cpp:76
         choose_scheme(ABHSF_POLICY_MSMF, (uint64_t)s, z, cost_b, is_binary, &bits);
-        msmf = bits + 2;        // block_statistics/mmf.cpp:93-96
+        v[1] = bits + 2;         // block_statistics/mmf.cpp:93-96
     }
-    pre = wave_sum_u64(pre);
-    msmf = wave_sum_u64(msmf);
 #pragma unroll
-    for (int q = 0; q < 4; q++) ev[q] = wave_sum_u64(ev[q]);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&totals[0], pre);
-        atomicAdd(&totals[1], msmf);
+    for (int q = 0; q < 6; q++) v[q] = wave_sum_u64(v[q]);
+    if ((threadIdx.x & 63) == 0)
 #pragma unroll
-        for (int q = 0; q < 4; q++) atomicAdd(&totals[2 + q], ev[q]);
+        for (int q = 0; q < 6; q++) red[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        unsigned long long a = 0;
+        for (int w = 0; w < kT / kWave; w++) a += red[w][threadIdx.x];
+        partial[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
     }
 }
 
@@ -273,6 +277,96 @@ DevMat Storage::view() const
     return A;
 }
 
+SpmvLaunch Storage::launch() const
+{
+    SpmvLaunch L;
+    L.bcol_bytes = bcol_bytes;
+    L.val_bytes = val_bytes;
+    L.bal.light = bal_light.get<uint32_t>();
+    L.bal.units = bal_units.get<uint32_t>();
+    L.bal.heavy = bal_heavy.get<uint32_t>();
+    L.bal.part = bal_part.get<double>();
+    L.bal.n_light = n_light;
+    L.bal.n_units = n_units;
+    L.bal.n_heavy = n_heavy;
+    return L;
+}
+
+void Storage::build_balance(int64_t unit, hipStream_t st)
+{
+    bal_light.release();
+    bal_units.release();
+    bal_heavy.release();
+    bal_part.release();
+    n_light = n_units = n_heavy = 0;
+    if (unit <= 0 || M == 0) return;
+    std::vector<uint32_t> ep[4], sp[4];
+    for (int x = 0; x < 4; x++) {
+        ep[x].resize((size_t)M + 1);
+        sp[x].resize((size_t)M + 1);
+        ABHSF_HIP(hipMemcpyAsync(ep[x].data(), eptr[x].get(), ((size_t)M + 1) * 4, hipMemcpyDeviceToHost, st));
+        ABHSF_HIP(hipMemcpyAsync(sp[x].data(), sptr[x].get(), ((size_t)M + 1) * 4, hipMemcpyDeviceToHost, st));
+    }
+    ABHSF_HIP(hipStreamSynchronize(st));
+    const int64_t H = 2 * unit;
+    std::vector<uint32_t> light, units, heavy;
+    bool any = false;
+    for (int64_t I = 0; I < M && !any; I++) {
+        int64_t w = 0;
+        for (int x = 0; x < 4; x++) w += (int64_t)ep[x][I + 1] - ep[x][I];
+        any = w > H;
+    }
+    if (!any) return;
+    std::vector<uint16_t> zz[4];
+    for (int x = 0; x < 3; x++) {
+        zz[x].resize(nb[x]);
+        if (nb[x]) ABHSF_HIP(hipMemcpyAsync(zz[x].data(), z1[x].get(), nb[x] * 2, hipMemcpyDeviceToHost, st));
+    }
+    ABHSF_HIP(hipStreamSynchronize(st));
+    const int64_t ss = (int64_t)s * s;
+    light.reserve((size_t)M);
+    for (int64_t I = 0; I < M; I++) {
+        int64_t w = 0;
+        for (int x = 0; x < 4; x++) w += (int64_t)ep[x][I + 1] - ep[x][I];
+        if (w <= H) {
+            light.push_back((uint32_t)I);
+            continue;
+        }
+        const uint32_t first = (uint32_t)(units.size() / 4);
+        for (int x = 0; x < 4; x++) {
+            const uint32_t b0 = sp[x][I], b1 = sp[x][I + 1];
+            uint64_t e = ep[x][I], ue = e;
+            uint32_t ub = b0;
+            int64_t acc = 0;
+            for (uint32_t b = b0; b < b1; b++) {
+                const int64_t wz = (x == DENSE) ? ss : (int64_t)zz[x][b] + 1;
+                if (acc > 0 && acc + wz > unit) {
+                    units.insert(units.end(), {(uint32_t)x, ub, b, (uint32_t)ue});
+                    ub = b;
+                    ue = e;
+                    acc = 0;
+                }
+                acc += wz;
+                e += (uint64_t)wz;
+            }
+            if (ub < b1) units.insert(units.end(), {(uint32_t)x, ub, b1, (uint32_t)ue});
+        }
+        heavy.insert(heavy.end(), {(uint32_t)I, first, (uint32_t)(units.size() / 4 - first)});
+    }
+    n_light = (int64_t)light.size();
+    n_units = (int64_t)(units.size() / 4);
+    n_heavy = (int64_t)(heavy.size() / 3);
+    bal_light.alloc(light.size() * 4);
+    bal_units.alloc(units.size() * 4);
+    bal_heavy.alloc(heavy.size() * 4);
+    bal_part.alloc((size_t)n_units * (size_t)s * sizeof(double) * (val_bytes == 4 ? 2 : 1));
+    if (!light.empty())
+        ABHSF_HIP(hipMemcpyAsync(bal_light.get(), light.data(), light.size() * 4, hipMemcpyHostToDevice, st));
+    ABHSF_HIP(hipMemcpyAsync(bal_units.get(), units.data(), units.size() * 4, hipMemcpyHostToDevice, st));
+    ABHSF_HIP(hipMemcpyAsync(bal_heavy.get(), heavy.data(), heavy.size() * 4, hipMemcpyHostToDevice, st));
+    ABHSF_HIP(hipStreamSynchronize(st));
+}
+
 uint64_t Storage::value_bytes() const
 {
     uint64_t b = 0;
@@ -282,7 +376,8 @@ uint64_t Storage::value_bytes() const
 
 uint64_t Storage::stored_bytes() const
 {
-    uint64_t b = value_bytes() + coo_idx.bytes() + csr_rowptr.bytes() + csr_col.bytes() + bm_words.bytes();
+    uint64_t b = value_bytes() + coo_idx.bytes() + csr_rowptr.bytes() + csr_col.bytes() + bm_words.bytes() +
+                 bal_light.bytes() + bal_units.bytes() + bal_heavy.bytes();
     for (int x = 0; x < 4; x++) b += sptr[x].bytes() + eptr[x].bytes() + bcol[x].bytes() + z1[x].bytes();
     return b;
 }
@@ -334,6 +429,8 @@ void build_from_device_csr(Storage &S, const int64_t *rowptr, const int32_t *col
     DevBuf totals(6 * sizeof(unsigned long long));
     ABHSF_HIP(hipMemsetAsync(totals.get(), 0, totals.bytes(), st));
     unsigned long long htot[6] = {0, 0, 0, 0, 0, 0};
+    const uint64_t nwg = grid_for(nblk);
+    DevBuf partial(6 * nwg * sizeof(unsigned long long));
     if (nnz) {
         k_block_starts<<<grid_for(nnz), kT, 0, st>>>(key.get<uint64_t>(), flag.get<uint32_t>(),
                                                      excl.get<uint32_t>(), nnz, nblk, bstart.get<uint32_t>(),
@@ -342,10 +439,14 @@ void build_from_device_csr(Storage &S, const int64_t *rowptr, const int32_t *col
         Four<uint32_t *> c4, e4;
         for (int x = 0; x < 4; x++) { c4.p[x] = cnt[x].get<uint32_t>(); e4.p[x] = ecnt[x].get<uint32_t>(); }
         const uint64_t hdr = ceil_log2_u64((uint64_t)M) + ceil_log2_u64((uint64_t)S.N) + 2;  // test.cpp:49
-        k_block_scheme<<<grid_for(nblk), kT, 0, st>>>(bstart.get<uint32_t>(), nblk, S.s, S.policy, S.cost_b,
-                                                      S.is_binary, hdr, sch.get<uint8_t>(), c4, e4,
-                                                      totals.get<unsigned long long>());
+        k_block_scheme<<<(unsigned)nwg, kT, 0, st>>>(bstart.get<uint32_t>(), nblk, S.s, S.policy, S.cost_b,
+                                                     S.is_binary, hdr, sch.get<uint8_t>(), c4, e4,
+                                                     partial.get<unsigned long long>());
         ABHSF_HIP(hipGetLastError());
+        for (int q = 0; q < 6; q++) {
+            uint64_t *pq = partial.get<uint64_t>() + (uint64_t)q * nwg;
+            exclusive_scan_u64(pq, pq, nwg, totals.get<uint64_t>() + q, st);
+        }
         ABHSF_HIP(hipMemcpyAsync(htot, totals.get(), sizeof(htot), hipMemcpyDeviceToHost, st));
         ABHSF_HIP(hipStreamSynchronize(st));
     }

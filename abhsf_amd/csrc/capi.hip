@@ -56,6 +56,15 @@ void reset_storage(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz)
     S.cost_b = h->opts.is_binary ? 0 : (uint64_t)S.val_bytes * 8;
 }
 
+// Work decomposition for the single-RHS kernel: units of about 2048 entries
+// (at least 8 rows' worth), heavy strips above twice that.
+int64_t balance_unit(const abhsf::Storage &S) { return std::max<int64_t>(2048, 8 * (int64_t)S.s); }
+
+void finish_build(abhsf_handle *h)
+{
+    if (h->opts.variant != ABHSF_VARIANT_STRIP) h->S.build_balance(balance_unit(h->S), h->stream);
+}
+
 void check_shape(int64_t m, int64_t n, int64_t nnz)
 {
     ABHSF_REQUIRE(m >= 1 && n >= 1 && m <= INT32_MAX && n <= INT32_MAX, ABHSF_ERR_INVALID,
@@ -110,6 +119,7 @@ abhsf_status abhsf_build_csr(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
         reset_storage(h, m, n, nnz);
         if (on_device) {
             abhsf::build_from_device_csr(h->S, rowptr, col, val, h->stream);
+            finish_build(h);
             return;
         }
         ABHSF_REQUIRE(rowptr[0] == 0 && rowptr[m] == nnz, ABHSF_ERR_INVALID, "row pointers do not span nnz");
@@ -119,6 +129,7 @@ abhsf_status abhsf_build_csr(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
         if (val && nnz) ABHSF_HIP(hipMemcpyAsync(dv.get(), val, dv.bytes(), hipMemcpyHostToDevice, h->stream));
         abhsf::build_from_device_csr(h->S, drp.get<int64_t>(), dc.get<int32_t>(), val ? dv.get<double>() : nullptr,
                                      h->stream);
+        finish_build(h);
     });
 }
 
@@ -160,6 +171,7 @@ abhsf_status abhsf_build_coo(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
         if (vals && nnz) ABHSF_HIP(hipMemcpyAsync(dv.get(), v.data(), dv.bytes(), hipMemcpyHostToDevice, h->stream));
         abhsf::build_from_device_csr(h->S, drp.get<int64_t>(), dc.get<int32_t>(), vals ? dv.get<double>() : nullptr,
                                      h->stream);
+        finish_build(h);
     });
 }
 

@@ -103,6 +103,60 @@ __device__ inline double group_sum(double v)
     return v;
 }
 
+// ---- DPP cross-lane moves (gfx9 encodings; rows are 16 lanes) ---------------
+enum : int {
+    DPP_QUAD_XOR1 = 0xB1,   // quad_perm [1,0,3,2]
+    DPP_QUAD_XOR2 = 0x4E,   // quad_perm [2,3,0,1]
+    DPP_ROW_SHR1 = 0x111,
+    DPP_ROW_SHR2 = 0x112,
+    DPP_ROW_SHR4 = 0x114,
+    DPP_ROW_SHR8 = 0x118,
+    DPP_ROW_MIRROR = 0x140,       // lane i <- 15 - i within a row
+    DPP_ROW_HALF_MIRROR = 0x141,  // lane i <- 7 - i within 8 lanes
+    DPP_ROW_BCAST15 = 0x142,      // lane 15 of row r -> row r+1
+    DPP_ROW_BCAST31 = 0x143       // lane 31 -> rows 2 and 3
+};
+
+// Lanes whose source is masked off or out of the row read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_f64(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK>((uint32_t)b), hi = dpp_u32<CTRL, ROW_MASK>((uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Inclusive prefix sum over the 64 lanes (Hillis-Steele within rows, then
+// row_bcast15 / row_bcast31 carry the row totals up the wave).
+__device__ inline uint32_t wave_incl_scan_dpp(uint32_t v)
+{
+    v += dpp_u32<DPP_ROW_SHR1>(v);
+    v += dpp_u32<DPP_ROW_SHR2>(v);
+    v += dpp_u32<DPP_ROW_SHR4>(v);
+    v += dpp_u32<DPP_ROW_SHR8>(v);
+    v += dpp_u32<DPP_ROW_BCAST15, 0xa>(v);
+    v += dpp_u32<DPP_ROW_BCAST31, 0xc>(v);
+    return v;
+}
+
+// Sum over aligned groups of G consecutive lanes (G in 1..16); every lane of
+// a group receives the group sum.
+template <int G>
+__device__ inline double group_sum_dpp(double v)
+{
+    if constexpr (G >= 2) v += dpp_f64<DPP_QUAD_XOR1>(v);
+    if constexpr (G >= 4) v += dpp_f64<DPP_QUAD_XOR2>(v);
+    if constexpr (G >= 8) v += dpp_f64<DPP_ROW_HALF_MIRROR>(v);
+    if constexpr (G >= 16) v += dpp_f64<DPP_ROW_MIRROR>(v);
+    return v;
+}
+
 __device__ inline uint64_t lanemask_lt() { return (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id())); }
 
 // Error-free transformation a + b = s + e (Knuth TwoSum, TAOCP vol. 2 4.2.2).

@@ -27,6 +27,8 @@
 // "Accurate sum and dot product", SIAM J. Sci. Comput. 26(6), 2005, Dot2).
 #include "spmv.hpp"
 
+#include <type_traits>
+
 namespace abhsf {
 namespace {
 
@@ -61,6 +63,296 @@ __device__ inline double prod(double v, double x, double &err)
     return p;
 }
 
+// ===========================================================================
+// Single right-hand side (SpMV).
+// ===========================================================================
+
+// Accumulate p (with product error pe) into a lane-private compensated pair.
+template <bool COMP>
+__device__ inline void acc_add(double &hi, double &lo, double v, double xv)
+{
+    if constexpr (COMP) {
+        double pe;
+        const double p = prod<true>(v, xv, pe);
+        double s, e;
+        two_sum(hi, p, s, e);
+        hi = s;
+        lo += e + pe;
+    } else {
+        hi = __builtin_fma(v, xv, hi);
+    }
+}
+
+// Sum (hi, lo) over aligned groups of P lanes (P in {1, 2, 4, 8}).
+template <bool COMP, int P>
+__device__ inline void group_reduce_p(double &hi, double &lo)
+{
+    if constexpr (!COMP) {
+        hi = group_sum_dpp<P>(hi);
+    } else {
+        auto step = [&](auto ctrl_tag) {
+            constexpr int CTRL = decltype(ctrl_tag)::value;
+            const double th = dpp_f64<CTRL>(hi), tl = dpp_f64<CTRL>(lo);
+            double s, e;
+            two_sum(hi, th, s, e);
+            hi = s;
+            lo += tl + e;
+        };
+        if constexpr (P >= 2) step(std::integral_constant<int, DPP_QUAD_XOR1>{});
+        if constexpr (P >= 4) step(std::integral_constant<int, DPP_QUAD_XOR2>{});
+        if constexpr (P >= 8) step(std::integral_constant<int, DPP_ROW_HALF_MIRROR>{});
+    }
+}
+
+template <bool COMP>
+__device__ inline void group_reduce(int P, double &hi, double &lo)
+{
+    switch (P) {
+    case 1: break;
+    case 2: group_reduce_p<COMP, 2>(hi, lo); break;
+    case 4: group_reduce_p<COMP, 4>(hi, lo); break;
+    default: group_reduce_p<COMP, 8>(hi, lo); break;
+    }
+}
+
+// COO or CSR group of strip I, element-parallel (one entry per lane per step).
+// Blocks [b0, b1) of scheme sc, whose first value is at offset e.
+template <typename VT, typename BT, typename IT, typename RT, bool COMP>
+__device__ inline void coo_csr_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1, uint32_t e,
+                                  const double *__restrict__ x, const Acc &acc, uint32_t *s_end, uint32_t *s_bc,
+                                  int lane)
+{
+    if (b0 >= b1) return;
+    const int s = A.s, k = A.k;
+    const uint32_t smask = (uint32_t)s - 1;
+    const VT *__restrict__ val = static_cast<const VT *>(A.val[sc]);
+#pragma unroll 1
+    for (uint32_t cb = b0; cb < b1; cb += kWave) {
+        const uint32_t jb = cb + lane;
+        const bool ok = jb < b1;
+        const uint32_t z = ok ? (uint32_t)A.z1[sc][jb] + 1u : 0u;
+        const uint32_t incl = wave_incl_scan_dpp(z);
+        const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        s_end[lane] = incl;
+        s_bc[lane] = ok ? (uint32_t)ld<BT>(A.bcol[sc], jb) : 0u;
+        wave_sync();
+        const int nbk = (int)min<uint32_t>(kWave, b1 - cb);
+#pragma unroll 1
+        for (uint32_t q = 0; q < T; q += kWave) {
+            const uint32_t t = q + lane;
+            if (t < T) {
+                int lo = 0, hi = nbk - 1;  // first chunk block with end > t
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_end[mid] > t) hi = mid; else lo = mid + 1;
+                }
+                uint32_t lr, lc;
+                if (sc == COO) {
+                    const uint32_t id = ld<IT>(A.coo_idx, e + t);
+                    lr = id >> k;
+                    lc = id & smask;
+                } else {
+                    const uint32_t qb = t - (lo ? s_end[lo - 1] : 0u);
+                    const RT *rp = static_cast<const RT *>(A.csr_rowptr) + (uint64_t)(cb + lo) * (s + 1);
+                    int rl = 0, rh = s - 1;
+                    while (rl < rh) {
+                        const int mid = (rl + rh + 1) >> 1;
+                        if ((uint32_t)rp[mid] <= qb) rl = mid; else rh = mid - 1;
+                    }
+                    lr = (uint32_t)rl;
+                    lc = A.csr_col[e + t];
+                }
+                const double v = (double)val[e + t];
+                const double xv = x[(int64_t)s_bc[lo] * s + lc];
+                double pe;
+                const double p = prod<COMP>(v, xv, pe);
+                acc_atomic<COMP>(acc, lr, p, pe);
+            }
+        }
+        e += T;
+        wave_sync();
+    }
+}
+
+// Bitmap or dense group of strip I.  Each lane owns cw consecutive row-major
+// positions of the block (P = s / cw lanes per row); it accumulates its
+// products in registers, the P lanes of a row are summed with DPP, and the
+// first lane of the row adds the sum to the strip accumulator.
+template <typename VT, typename BT, bool COMP>
+__device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1, uint64_t vb,
+                               const double *__restrict__ x, const Acc &acc, int lane)
+{
+    if (b0 >= b1) return;
+    const int s = A.s, k = A.k;
+    const uint32_t ss = (uint32_t)s * (uint32_t)s;
+    const uint32_t cw = s >= 64 ? (uint32_t)s : (ss >= 64 ? ss / 64 : 1u);
+    const int npass = (int)((ss + 64u * cw - 1) / (64u * cw));
+    const int P = (int)((uint32_t)s / cw);
+    const int64_t n = A.n;
+    const VT *__restrict__ val = static_cast<const VT *>(A.val[sc]);
+#pragma unroll 1
+    for (uint32_t b = b0; b < b1; b++) {
+        const int64_t c0 = (int64_t)ld<BT>(A.bcol[sc], b) * s;
+        const uint64_t *__restrict__ words = A.bm_words + (uint64_t)b * A.W;
+        const uint64_t db = (uint64_t)b * ss;
+#pragma unroll 1
+        for (int q = 0; q < npass; q++) {
+            const uint32_t pbase = ((uint32_t)q * 64u + (uint32_t)lane) * cw;
+            const bool valid = pbase < ss;
+            const uint32_t row = pbase >> k, col0 = pbase & ((uint32_t)s - 1);
+            const int64_t gc0 = c0 + col0;
+            double hi = 0.0, lo = 0.0;
+            if (sc == DENSE) {
+                if (valid) {
+                    const VT *vp = val + db + pbase;
+                    if (gc0 + (int64_t)cw <= n && (cw & 1) == 0) {
+#pragma unroll 4
+                        for (uint32_t c = 0; c < cw; c += 2) {
+                            const double2 xv = *reinterpret_cast<const double2 *>(x + gc0 + c);
+                            double v0, v1;
+                            if constexpr (sizeof(VT) == 8) {
+                                const double2 vv = *reinterpret_cast<const double2 *>(vp + c);
+                                v0 = vv.x;
+                                v1 = vv.y;
+                            } else {
+                                const float2 vv = *reinterpret_cast<const float2 *>(vp + c);
+                                v0 = vv.x;
+                                v1 = vv.y;
+                            }
+                            acc_add<COMP>(hi, lo, v0, xv.x);
+                            acc_add<COMP>(hi, lo, v1, xv.y);
+                        }
+                    } else {
+                        for (uint32_t c = 0; c < cw; c++)
+                            if (gc0 + c < n) acc_add<COMP>(hi, lo, (double)vp[c], x[gc0 + c]);
+                    }
+                }
+            } else {
+                // bitmap: popcount of my positions -> value offset by a wave scan
+                const uint32_t nw = cw > 64 ? cw / 64 : 1;
+                const uint32_t w0 = pbase >> 6, sh = pbase & 63;
+                const uint64_t m = cw >= 64 ? ~0ull : ((1ull << cw) - 1ull);
+                uint32_t cnt = 0;
+                if (valid)
+                    for (uint32_t w = 0; w < nw; w++) cnt += __popcll((words[w0 + w] >> sh) & m);
+                const uint32_t incl = wave_incl_scan_dpp(cnt);
+                const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+                uint64_t off = vb + incl - cnt;
+                if (valid) {
+                    for (uint32_t w = 0; w < nw; w++) {
+                        uint64_t bits = (words[w0 + w] >> sh) & m;
+                        while (bits) {
+                            const uint32_t c = (uint32_t)__builtin_ctzll(bits) + w * 64;
+                            bits &= bits - 1;
+                            acc_add<COMP>(hi, lo, (double)val[off++], x[gc0 + c]);
+                        }
+                    }
+                }
+                vb += tot;
+            }
+            group_reduce<COMP>(P, hi, lo);
+            if (valid && (lane & (P - 1)) == 0) {
+                if constexpr (COMP) {
+                    double s2, e2;
+                    two_sum(acc.hi[row], hi, s2, e2);
+                    acc.hi[row] = s2;
+                    acc.lo[row] += lo + e2;
+                } else {
+                    acc.hi[row] += hi;
+                }
+            }
+        }
+    }
+}
+
+// Work items: items [0, n_light) are whole strips (light[item], or the item
+// itself when light is null) whose rows are written to y; items n_light + u are
+// units u of heavy strips: one scheme group's block range, whose partial sums
+// go to part[u] and are combined by k_spmv_fixup.
+template <typename VT, typename BT, typename IT, typename RT, bool COMP>
+__global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__restrict__ x, double *__restrict__ y,
+                                                    double alpha, double beta, Balance B)
+{
+    constexpr int kWaves = 4;
+    __shared__ double s_acc[kWaves][256 * (COMP ? 2 : 1)];
+    __shared__ uint32_t s_end[kWaves][kWave];
+    __shared__ uint32_t s_bc[kWaves][kWave];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * kWaves + wave;
+    if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
+    const int s = A.s;
+    Acc acc;
+    acc.hi = s_acc[wave];
+    acc.lo = COMP ? s_acc[wave] + 256 : nullptr;
+    for (int i = lane; i < s; i += kWave) {
+        acc.hi[i] = 0.0;
+        if constexpr (COMP) acc.lo[i] = 0.0;
+    }
+    wave_sync();
+    if (item < B.n_light) {
+        const int64_t I = B.light ? (int64_t)B.light[item] : item;
+        coo_csr_r1<VT, BT, IT, RT, COMP>(A, COO, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc,
+                                         s_end[wave], s_bc[wave], lane);
+        coo_csr_r1<VT, BT, IT, RT, COMP>(A, CSR, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc,
+                                         s_end[wave], s_bc[wave], lane);
+        wave_sync();
+        bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x, acc, lane);
+        bmdn_r1<VT, BT, COMP>(A, DENSE, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], 0, x, acc, lane);
+        wave_sync();
+        for (int r = lane; r < s; r += kWave) {
+            const int64_t gr = I * s + r;
+            if (gr >= A.m) break;
+            double a = acc.hi[r];
+            if constexpr (COMP) a += acc.lo[r];
+            y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * y[gr];
+        }
+    } else {
+        const int64_t u = item - B.n_light;
+        const uint32_t *U = B.units + u * 4;
+        const uint32_t sc = U[0] & 3u, b0 = U[1], b1 = U[2], e0 = U[3];
+        if (sc <= CSR)
+            coo_csr_r1<VT, BT, IT, RT, COMP>(A, (int)sc, b0, b1, e0, x, acc, s_end[wave], s_bc[wave], lane);
+        else
+            bmdn_r1<VT, BT, COMP>(A, (int)sc, b0, b1, e0, x, acc, lane);
+        wave_sync();
+        double *part = B.part + u * (size_t)s * (COMP ? 2 : 1);
+        for (int r = lane; r < s; r += kWave) {
+            part[r] = acc.hi[r];
+            if constexpr (COMP) part[s + r] = acc.lo[r];
+        }
+    }
+}
+
+// Rows of heavy strips: sum the units' partials in unit order (deterministic).
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_spmv_fixup(int s, int64_t m, double *__restrict__ y, double alpha,
+                                                    double beta, Balance B)
+{
+    const int64_t h = blockIdx.x;
+    const uint32_t I = B.heavy[h * 3], u0 = B.heavy[h * 3 + 1], nu = B.heavy[h * 3 + 2];
+    for (int r = threadIdx.x; r < s; r += blockDim.x) {
+        const int64_t gr = (int64_t)I * s + r;
+        if (gr >= m) break;
+        double hi = 0.0, lo = 0.0;
+        for (uint32_t u = u0; u < u0 + nu; u++) {
+            const double *part = B.part + (size_t)u * s * (COMP ? 2 : 1);
+            if constexpr (COMP) {
+                double s2, e2;
+                two_sum(hi, part[r], s2, e2);
+                hi = s2;
+                lo += part[s + r] + e2;
+            } else {
+                hi += part[r];
+            }
+        }
+        const double a = hi + lo;
+        y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * y[gr];
+    }
+}
+
+// ===========================================================================
+// Several right-hand sides (SpMM), R lanes per entry.
+// ===========================================================================
 template <typename VT, typename BT, typename IT, typename RT, int R, bool COMP>
 __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__restrict__ X, int64_t ldx,
                                                     double *__restrict__ Y, int64_t ldy, int nrhs,
@@ -243,11 +535,34 @@ void launch_t(const DevMat &A, const double *X, int64_t ldx, double *Y, int64_t 
 }
 
 template <typename VT, typename BT, typename IT, typename RT, bool COMP>
-void launch_r(int R, const DevMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy, int nrhs,
-              double alpha, double beta, hipStream_t st)
+void launch_spmv(const DevMat &A, const Balance &B, const double *x, double *y, double alpha, double beta,
+                 hipStream_t st)
+{
+    if (A.M == 0) return;
+    Balance b = B;
+    if (!b.light && b.n_units == 0) b.n_light = A.M;  // one item per strip
+    const int64_t nitems = b.n_light + b.n_units;
+    const int64_t nblk = (nitems + 3) / 4;
+    ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many strips");
+    hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP>), dim3((unsigned)nblk), dim3(256), 0, st, A, x, y, alpha,
+                       beta, b);
+    ABHSF_HIP(hipGetLastError());
+    if (b.n_heavy) {
+        hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)b.n_heavy), dim3(A.s < 256 ? 64 : 256), 0, st, A.s,
+                           A.m, y, alpha, beta, b);
+        ABHSF_HIP(hipGetLastError());
+    }
+}
+
+template <typename VT, typename BT, typename IT, typename RT, bool COMP>
+void launch_r(int R, const DevMat &A, const Balance &Bal, const double *X, int64_t ldx, double *Y, int64_t ldy,
+              int nrhs, double alpha, double beta, hipStream_t st)
 {
     switch (R) {
-    case 1: launch_t<VT, BT, IT, RT, 1, COMP>(A, X, ldx, Y, ldy, nrhs, alpha, beta, st); break;
+    case 1:
+        if (nrhs == 1 && ldx == 1 && ldy == 1) launch_spmv<VT, BT, IT, RT, COMP>(A, Bal, X, Y, alpha, beta, st);
+        else launch_t<VT, BT, IT, RT, 1, COMP>(A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        break;
     case 2: launch_t<VT, BT, IT, RT, 2, COMP>(A, X, ldx, Y, ldy, nrhs, alpha, beta, st); break;
     case 4: launch_t<VT, BT, IT, RT, 4, COMP>(A, X, ldx, Y, ldy, nrhs, alpha, beta, st); break;
     case 8: launch_t<VT, BT, IT, RT, 8, COMP>(A, X, ldx, Y, ldy, nrhs, alpha, beta, st); break;
@@ -261,17 +576,18 @@ template <typename VT, bool COMP>
 void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int64_t ldx, double *Y, int64_t ldy,
               int nrhs, double alpha, double beta, hipStream_t st)
 {
+    const Balance &Bal = L.bal;
     const bool b16 = L.bcol_bytes == 2;
     const int s = A.s;
     if (s <= 8) {
-        if (b16) launch_r<VT, uint16_t, uint8_t, uint8_t, COMP>(R, A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
-        else launch_r<VT, uint32_t, uint8_t, uint8_t, COMP>(R, A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        if (b16) launch_r<VT, uint16_t, uint8_t, uint8_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        else launch_r<VT, uint32_t, uint8_t, uint8_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
     } else if (s <= 16) {
-        if (b16) launch_r<VT, uint16_t, uint8_t, uint16_t, COMP>(R, A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
-        else launch_r<VT, uint32_t, uint8_t, uint16_t, COMP>(R, A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        if (b16) launch_r<VT, uint16_t, uint8_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        else launch_r<VT, uint32_t, uint8_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
     } else {
-        if (b16) launch_r<VT, uint16_t, uint16_t, uint16_t, COMP>(R, A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
-        else launch_r<VT, uint32_t, uint16_t, uint16_t, COMP>(R, A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        if (b16) launch_r<VT, uint16_t, uint16_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        else launch_r<VT, uint32_t, uint16_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
     }
 }
 

@@ -5,9 +5,21 @@
 
 namespace abhsf {
 
+// Work decomposition of the single-RHS kernel: light strips are whole work
+// items; heavy strips are cut into units (one scheme group's block range each)
+// whose partial row sums are combined by a fix-up pass.
+struct Balance {
+    const uint32_t *light = nullptr;  // [n_light] strip ids (null: every strip, in order)
+    const uint32_t *units = nullptr;  // [n_units][4] {scheme, b0, b1, value offset of b0}
+    const uint32_t *heavy = nullptr;  // [n_heavy][3] {strip, first unit, unit count}
+    double *part = nullptr;           // [n_units][s] (x2 when compensated)
+    int64_t n_light = 0, n_units = 0, n_heavy = 0;
+};
+
 struct SpmvLaunch {
     int bcol_bytes;  // 2 or 4
     int val_bytes;   // 8 (f64) or 4 (f32, compensated accumulation)
+    Balance bal;
 };
 
 // Largest power-of-two rhs tile whose s x R accumulator fits the LDS budget.

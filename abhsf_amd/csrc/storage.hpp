@@ -18,12 +18,18 @@ struct Storage {
     uint64_t nb[4] = {0, 0, 0, 0}, ne[4] = {0, 0, 0, 0};
     DevBuf sptr[4], eptr[4], bcol[4], z1[4], val[4];
     DevBuf coo_idx, csr_rowptr, csr_col, bm_words;
+    // work decomposition of the single-RHS kernel (heavy strips split in units)
+    DevBuf bal_light, bal_units, bal_heavy, bal_part;
+    int64_t n_light = 0, n_units = 0, n_heavy = 0;
     // footprints (bits) under both cost models
     uint64_t smf_preproc = 0, smf_msmf = 0;
     double build_ms = 0.0;
 
     DevMat view() const;
-    SpmvLaunch launch() const { return SpmvLaunch{bcol_bytes, val_bytes}; }
+    SpmvLaunch launch() const;
+    // Split strips whose work exceeds 2*unit entries into units of about
+    // `unit` entries (unit <= 0: no splitting).
+    void build_balance(int64_t unit, hipStream_t st);
     uint64_t stored_bytes() const;
     uint64_t value_bytes() const;
     void set_shape(int64_t m_, int64_t n_, int s_);
