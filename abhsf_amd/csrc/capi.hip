@@ -7,6 +7,22 @@
 #include <numeric>
 #include <vector>
 
+extern "C" abhsf_status abhsf_block_hist_impl(const int64_t *rowptr, const int32_t *col, int64_t m, int64_t n,
+                                              int64_t nnz, int32_t on_device, int32_t kr, int32_t ks,
+                                              int64_t strip_stride, uint64_t *hist, int64_t hist_cap,
+                                              uint64_t *nblocks, uint64_t *sampled_nnz, void *stream,
+                                              std::string *err);
+
+extern "C" abhsf_status abhsf_mlp_forward_impl(const float *theta, int32_t F, const float *X, int64_t B, float *H1,
+                                               float *H2, float *out, void *stream, std::string *err);
+extern "C" abhsf_status abhsf_mlp_backward_impl(const float *theta, int32_t F, const float *X, int64_t B,
+                                                const float *H1, const float *H2, const float *out, const float *y,
+                                                const float *w, float wsum, float *D1, float *D2, float *D3,
+                                                float *grad, void *stream, std::string *err);
+extern "C" abhsf_status abhsf_mlp_train_impl(float *theta, int32_t F, const float *X, const float *y, const float *w,
+                                             int64_t B, int64_t steps, float lr, float *loss_host, void *stream,
+                                             std::string *err);
+
 struct abhsf_handle {
     abhsf_options opts;
     hipStream_t stream;
@@ -241,6 +257,51 @@ abhsf_status abhsf_export(const abhsf_handle *h, const char *name, void *dst, si
             ABHSF_HIP(hipStreamSynchronize(h->stream));
         }
     });
+}
+
+abhsf_status abhsf_block_hist(const int64_t *rowptr, const int32_t *col, int64_t m, int64_t n, int64_t nnz,
+                              int32_t on_device, int32_t kr, int32_t ks, int64_t strip_stride, uint64_t *hist,
+                              int64_t hist_cap, uint64_t *nblocks, uint64_t *sampled_nnz, void *stream)
+{
+    std::string err;
+    const abhsf_status st = abhsf_block_hist_impl(rowptr, col, m, n, nnz, on_device, kr, ks, strip_stride, hist,
+                                                  hist_cap, nblocks, sampled_nnz, stream, &err);
+    if (st != ABHSF_OK) return fail(st, err);
+    g_last_error.clear();
+    return ABHSF_OK;
+}
+
+abhsf_status abhsf_mlp_forward(const float *theta, int32_t nfeat, const float *X, int64_t nsamples, float *H1,
+                               float *H2, float *out, void *stream)
+{
+    std::string err;
+    const abhsf_status st = abhsf_mlp_forward_impl(theta, nfeat, X, nsamples, H1, H2, out, stream, &err);
+    if (st != ABHSF_OK) return fail(st, err);
+    g_last_error.clear();
+    return ABHSF_OK;
+}
+
+abhsf_status abhsf_mlp_backward(const float *theta, int32_t nfeat, const float *X, int64_t nsamples,
+                                const float *H1, const float *H2, const float *out, const float *y,
+                                const float *w, float wsum, float *D1, float *D2, float *D3, float *grad,
+                                void *stream)
+{
+    std::string err;
+    const abhsf_status st = abhsf_mlp_backward_impl(theta, nfeat, X, nsamples, H1, H2, out, y, w, wsum, D1, D2, D3,
+                                                    grad, stream, &err);
+    if (st != ABHSF_OK) return fail(st, err);
+    g_last_error.clear();
+    return ABHSF_OK;
+}
+
+abhsf_status abhsf_mlp_train(float *theta, int32_t nfeat, const float *X, const float *y, const float *w,
+                             int64_t nsamples, int64_t steps, float lr, float *loss_host, void *stream)
+{
+    std::string err;
+    const abhsf_status st = abhsf_mlp_train_impl(theta, nfeat, X, y, w, nsamples, steps, lr, loss_host, stream, &err);
+    if (st != ABHSF_OK) return fail(st, err);
+    g_last_error.clear();
+    return ABHSF_OK;
 }
 
 abhsf_status abhsf_synchronize(abhsf_handle *h)

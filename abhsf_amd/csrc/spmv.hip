@@ -174,6 +174,82 @@ __device__ inline void coo_csr_r1(const DevMat &A, int sc, uint32_t b0, uint32_t
     }
 }
 
+// COO group, blocks [b0, b1) with first value at offset e.  Chunks of 64
+// blocks, lane j <-> block j.  Phase A: every lane handles the first ZS
+// entries of its own block (independent loads, no lookup); phase B, only when
+// some block of the chunk holds more than ZS entries: the remaining entries
+// are spread one per lane and located by a binary search over the chunk's
+// prefix sums in LDS.  The next chunk's block metadata is loaded ahead.
+template <typename VT, typename BT, typename IT, bool COMP, int ZS>
+__device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_t e, const double *__restrict__ x,
+                              const Acc &acc, uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, int lane)
+{
+    if (b0 >= b1) return;
+    const int s = A.s, k = A.k;
+    const uint32_t smask = (uint32_t)s - 1;
+    const VT *__restrict__ val = static_cast<const VT *>(A.val[COO]);
+    const IT *__restrict__ idx = static_cast<const IT *>(A.coo_idx);
+    const uint16_t *__restrict__ z1 = A.z1[COO];
+    const BT *__restrict__ bcol = static_cast<const BT *>(A.bcol[COO]);
+    uint32_t jb = b0 + lane;
+    uint32_t z = jb < b1 ? (uint32_t)z1[jb] + 1u : 0u;
+    uint32_t bc = jb < b1 ? (uint32_t)bcol[jb] : 0u;
+#pragma unroll 1
+    for (uint32_t cb = b0; cb < b1; cb += kWave) {
+        const uint32_t jn = cb + kWave + (uint32_t)lane;
+        const bool okn = jn < b1;
+        const uint32_t zn = okn ? (uint32_t)z1[jn] + 1u : 0u;
+        const uint32_t bcn = okn ? (uint32_t)bcol[jn] : 0u;
+        const uint32_t incl = wave_incl_scan_dpp(z);
+        const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        const uint32_t st = e + incl - z;
+        const int64_t xb = (int64_t)bc * s;
+#pragma unroll
+        for (int i = 0; i < ZS; i++) {
+            if ((uint32_t)i < z) {
+                const uint32_t id = idx[st + i];
+                const double v = (double)val[st + i];
+                const double xv = x[xb + (id & smask)];
+                double pe;
+                const double p = prod<COMP>(v, xv, pe);
+                acc_atomic<COMP>(acc, id >> k, p, pe);
+            }
+        }
+        const uint32_t rem = z > (uint32_t)ZS ? z - (uint32_t)ZS : 0u;
+        if (__builtin_amdgcn_read_exec() & __ballot(rem != 0)) {
+            const uint32_t rincl = wave_incl_scan_dpp(rem);
+            const uint32_t RT = (uint32_t)__builtin_amdgcn_readlane((int)rincl, kWave - 1);
+            s_end[lane] = rincl;
+            s_bc[lane] = bc;
+            s_st[lane] = st + (uint32_t)ZS;
+            wave_sync();
+            const int nbk = (int)min<uint32_t>(kWave, b1 - cb);
+#pragma unroll 1
+            for (uint32_t q = 0; q < RT; q += kWave) {
+                const uint32_t t = q + lane;
+                if (t < RT) {
+                    int lo = 0, hi = nbk - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_end[mid] > t) hi = mid; else lo = mid + 1;
+                    }
+                    const uint32_t pos = s_st[lo] + t - (lo ? s_end[lo - 1] : 0u);
+                    const uint32_t id = idx[pos];
+                    const double v = (double)val[pos];
+                    const double xv = x[(int64_t)s_bc[lo] * s + (id & smask)];
+                    double pe;
+                    const double p = prod<COMP>(v, xv, pe);
+                    acc_atomic<COMP>(acc, id >> k, p, pe);
+                }
+            }
+            wave_sync();
+        }
+        e += T;
+        z = zn;
+        bc = bcn;
+    }
+}
+
 // Bitmap or dense group of strip I.  Each lane owns cw consecutive row-major
 // positions of the block (P = s / cw lanes per row); it accumulates its
 // products in registers, the P lanes of a row are summed with DPP, and the
@@ -241,10 +317,30 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
                 if (valid) {
                     for (uint32_t w = 0; w < nw; w++) {
                         uint64_t bits = (words[w0 + w] >> sh) & m;
+                        const int64_t xb = gc0 + (int64_t)w * 64;
+                        // four set bits per step so their loads are in flight together
                         while (bits) {
-                            const uint32_t c = (uint32_t)__builtin_ctzll(bits) + w * 64;
+                            const uint32_t c0 = (uint32_t)__builtin_ctzll(bits);
                             bits &= bits - 1;
-                            acc_add<COMP>(hi, lo, (double)val[off++], x[gc0 + c]);
+                            const bool h1 = bits != 0;
+                            const uint32_t c1 = h1 ? (uint32_t)__builtin_ctzll(bits) : c0;
+                            bits &= bits - 1;
+                            const bool h2 = bits != 0;
+                            const uint32_t c2 = h2 ? (uint32_t)__builtin_ctzll(bits) : c0;
+                            bits &= bits - 1;
+                            const bool h3 = bits != 0;
+                            const uint32_t c3 = h3 ? (uint32_t)__builtin_ctzll(bits) : c0;
+                            bits &= bits - 1;
+                            const double v0 = (double)val[off];
+                            const double v1 = (double)val[h1 ? off + 1 : off];
+                            const double v2 = (double)val[h2 ? off + 2 : off];
+                            const double v3 = (double)val[h3 ? off + 3 : off];
+                            const double x0 = x[xb + c0], x1 = x[xb + c1], x2 = x[xb + c2], x3 = x[xb + c3];
+                            acc_add<COMP>(hi, lo, v0, x0);
+                            if (h1) acc_add<COMP>(hi, lo, v1, x1);
+                            if (h2) acc_add<COMP>(hi, lo, v2, x2);
+                            if (h3) acc_add<COMP>(hi, lo, v3, x3);
+                            off += 1u + h1 + h2 + h3;
                         }
                     }
                 }
@@ -277,6 +373,7 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     __shared__ double s_acc[kWaves][256 * (COMP ? 2 : 1)];
     __shared__ uint32_t s_end[kWaves][kWave];
     __shared__ uint32_t s_bc[kWaves][kWave];
+    __shared__ uint32_t s_st[kWaves][kWave];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t item = (int64_t)blockIdx.x * kWaves + wave;
     if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
@@ -291,8 +388,8 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     wave_sync();
     if (item < B.n_light) {
         const int64_t I = B.light ? (int64_t)B.light[item] : item;
-        coo_csr_r1<VT, BT, IT, RT, COMP>(A, COO, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc,
-                                         s_end[wave], s_bc[wave], lane);
+        coo_r1<VT, BT, IT, COMP, 2>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end[wave],
+                                    s_bc[wave], s_st[wave], lane);
         coo_csr_r1<VT, BT, IT, RT, COMP>(A, CSR, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc,
                                          s_end[wave], s_bc[wave], lane);
         wave_sync();
@@ -310,7 +407,9 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
         const int64_t u = item - B.n_light;
         const uint32_t *U = B.units + u * 4;
         const uint32_t sc = U[0] & 3u, b0 = U[1], b1 = U[2], e0 = U[3];
-        if (sc <= CSR)
+        if (sc == COO)
+            coo_r1<VT, BT, IT, COMP, 2>(A, b0, b1, e0, x, acc, s_end[wave], s_bc[wave], s_st[wave], lane);
+        else if (sc == CSR)
             coo_csr_r1<VT, BT, IT, RT, COMP>(A, (int)sc, b0, b1, e0, x, acc, s_end[wave], s_bc[wave], lane);
         else
             bmdn_r1<VT, BT, COMP>(A, (int)sc, b0, b1, e0, x, acc, lane);

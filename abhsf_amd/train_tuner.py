@@ -1,0 +1,129 @@
+"""Train the learned tuner on the GPU and judge it against exhaustive search.
+
+  python -m abhsf_amd.train_tuner --out DIR [--steps 4000]
+
+1. corpus: seeded synthetic matrices of every family (train and held-out
+   sets use different seeds and sizes; the held-out set contains the R-MAT
+   scale-22 and block_mix n = 2^22 workloads);
+2. labels: exhaustive timing sweep of all (s, variant) candidates
+   (tuner.sweep) -> y = log(ms per entry);
+3. features: tuner.matrix_features (GPU block histograms on a strip sample);
+4. training: tuner.train (full-batch Adam by the library's MLP kernels);
+5. report: per held-out matrix, the regret of the MLP pick, of plain ABHSF
+   (s = 64) and of the storage-optimal block size (the reference's search,
+   block_statistics/mmf.h:77-102), all against the exhaustive best.
+Writes DIR/tuner_model.json, DIR/tuner_dataset.npz and DIR/tuner_report.json.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import hip, synth, tuner
+
+
+def corpus(split: str):
+    """(name, generator) pairs."""
+    if split == "train":
+        out = [(f"lap_{g}", (lambda g=g: synth.laplacian2d(g))) for g in (256, 700, 1500)]
+        out += [(f"stencil_{g}", (lambda g=g: synth.stencil27(g))) for g in (40, 80, 128)]
+        for n in (1 << 17, 1 << 19, 1 << 21):
+            for pd, pb in ((0.2, 0.3), (0.0, 0.0), (1.0, 0.0), (0.0, 1.0), (0.5, 0.5)):
+                out.append((f"mix_{n}_{pd}_{pb}", (lambda n=n, pd=pd, pb=pb: synth.block_mix(n, 11, pd, pb))))
+        for sc, ef in ((16, 8), (18, 16), (20, 8), (20, 16), (21, 16)):
+            out.append((f"rmat_{sc}_{ef}", (lambda sc=sc, ef=ef: synth.rmat(sc, ef, seed=11))))
+        for n, bw in ((100000, 4), (100000, 32), (1000000, 8), (1000000, 64)):
+            out.append((f"banded_{n}_{bw}", (lambda n=n, bw=bw: synth.banded(n, bw, 11))))
+        for n, bs, nbr, d in ((100000, 8, 4, 0.3), (100000, 16, 2, 0.1), (1000000, 4, 8, 0.5),
+                              (1000000, 32, 4, 0.05), (500000, 16, 8, 0.2)):
+            out.append((f"rblk_{n}_{bs}_{nbr}_{d}", (lambda n=n, bs=bs, nbr=nbr, d=d:
+                                                    synth.random_blocks(n, bs, nbr, d, 11))))
+        return out
+    return [
+        ("lap_1024", lambda: synth.laplacian2d(1024)),
+        ("stencil_100", lambda: synth.stencil27(100)),
+        ("mix_2^20_a", lambda: synth.block_mix(1 << 20, 5)),
+        ("mix_2^20_b", lambda: synth.block_mix(1 << 20, 6, 0.3, 0.2)),
+        ("rmat_19_16", lambda: synth.rmat(19, 16, seed=3)),
+        ("banded_300000_16", lambda: synth.banded(300000, 16, 3)),
+        ("rblk_300000_8_4_0.2", lambda: synth.random_blocks(300000, 8, 4, 0.2, 3)),
+        ("rmat_22_16", lambda: synth.rmat(22, 16, seed=0)),
+        ("block_mix_2^22", lambda: synth.block_mix(1 << 22, 0)),
+    ]
+
+
+def collect(split, stream, log):
+    X, y, names, times = [], [], [], []
+    for name, gen in corpus(split):
+        t0 = time.perf_counter()
+        A = gen()
+        d_rp, d_c = hip.DeviceArray.from_numpy(A.rowptr), hip.DeviceArray.from_numpy(A.col)
+        Xc, per_s = tuner.matrix_features(A, d_rp, d_c, stream)
+        tm = tuner.sweep(A, stream)
+        for i, cand in enumerate(tuner.candidates()):
+            X.append(Xc[i])
+            y.append(math.log(tm[cand] / A.nnz))
+        names.append(name)
+        times.append({f"{s}/{v}": tm[(s, v)] for s, v in tm} | {"nnz": A.nnz,
+                     "storage_opt": list(tuner.pick_storage_optimal(per_s))})
+        best = min(tm, key=tm.get)
+        log(f"{split} {name}: nnz={A.nnz} best={best} {tm[best]:.4f} ms "
+            f"s64={tm[(64, 'balanced')]:.4f} ms ({time.perf_counter() - t0:.1f} s)")
+    return np.array(X), np.array(y), names, times
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+
+    def log(msg):
+        print(f"[train_tuner] {msg}", flush=True)
+
+    stream = hip.Stream()
+    Xtr, ytr, ntr, ttr = collect("train", stream, log)
+    Xte, yte, nte, tte = collect("test", stream, log)
+    np.savez(os.path.join(args.out, "tuner_dataset.npz"), Xtr=Xtr, ytr=ytr, Xte=Xte, yte=yte)
+    model, losses = tuner.train(Xtr, ytr, steps=args.steps, lr=args.lr)
+    log(f"trained: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {model.meta['train_s']:.1f} s")
+    model.save(os.path.join(args.out, "tuner_model.json"))
+
+    cands = tuner.candidates()
+    nc = len(cands)
+    report = {"train_loss_first": float(losses[0]), "train_loss_last": float(losses[-1]), "matrices": []}
+    agg = {"mlp": [], "plain_s64": [], "storage_opt": []}
+    for i, name in enumerate(nte):
+        tm = {(s, v): tte[i][f"{s}/{v}"] for s, v in cands}
+        pred = model.predict(Xte[i * nc:(i + 1) * nc])
+        pk = cands[int(np.argmin(pred))]
+        so = tuple(tte[i]["storage_opt"])
+        row = {
+            "matrix": name, "nnz": tte[i]["nnz"], "best": list(min(tm, key=tm.get)),
+            "best_ms": min(tm.values()), "mlp_pick": list(pk), "mlp_ms": tm[pk],
+            "regret_mlp": tuner.regret(tm, pk), "regret_plain_s64": tuner.regret(tm, (64, "balanced")),
+            "regret_storage_opt": tuner.regret(tm, so), "plain_s64_ms": tm[(64, "balanced")],
+            "speedup_vs_plain": tm[(64, "balanced")] / tm[pk],
+        }
+        report["matrices"].append(row)
+        agg["mlp"].append(row["regret_mlp"])
+        agg["plain_s64"].append(row["regret_plain_s64"])
+        agg["storage_opt"].append(row["regret_storage_opt"])
+        log(f"holdout {name}: pick {pk} regret {row['regret_mlp']:.1f}% | plain s64 {row['regret_plain_s64']:.1f}% "
+            f"| storage-opt {so} {row['regret_storage_opt']:.1f}%")
+    report["mean_regret"] = {k: float(np.mean(v)) for k, v in agg.items()}
+    report["max_regret"] = {k: float(np.max(v)) for k, v in agg.items()}
+    json.dump(report, open(os.path.join(args.out, "tuner_report.json"), "w"), indent=1)
+    log(f"mean regret: {report['mean_regret']}")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,7 +1,309 @@
-"""Block-size / kernel-variant tuner (placeholder: exhaustive timing)."""
-NAME = "exhaustive-timing"
+"""Learned block-size / kernel-variant tuner.
+
+A 2 x 64 hidden-layer FP32 MLP (abhsf_amd/csrc/mlp.hip, trained and run on
+the GPU through the C ABI) predicts log(SpMV time per entry) for every
+candidate (s, variant) from cheap matrix statistics; the tuner picks the
+candidate with the smallest prediction.  Labels come from exhaustive timing
+sweeps of this library (`sweep`), which also judge the tuner's regret
+(block_statistics/consistency.cpp:147-149: (t_pick - t_min) / t_min * 100).
+
+Features (the statistics block_statistics/ computes, block_stats.cpp:78-100,
+174-221 and rows-nnz-stdev.cpp:11-30, taken on the GPU):
+  global      log2 n, log2 nnz, log2 mean row nnz, row-nnz stdev / mean,
+              log2 (max row nnz / mean), fraction of empty rows
+  per (s, v)  log2 s, log2 (blocks / entries), fraction of blocks and of
+              entries in each scheme, log2 predicted stored bytes per entry,
+              strip-work stdev / mean, log2 (max strip work / mean),
+              fraction of entries in strips above the split threshold,
+              log2 strips, variant flag
+Block-size histograms come from abhsf_block_hist on a sample of strips.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import capi, hip
+
+NAME = "mlp-2x64"
+S_CANDIDATES = (8, 16, 32, 64, 128, 256)
+VARIANTS = ("strip", "balanced")
+HIDDEN = 64
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_model.json")
+SAMPLE_NNZ = 2_000_000  # entries counted per block-size histogram
 
 
-def pick_block_size(A, candidates):
-    """Return the chosen s, or None to use the best measured candidate."""
-    return None
+# ---- cost model of the stored layout (mirrors Storage::stored_bytes) --------
+
+def _ceil_log2(x: int) -> int:
+    return 0 if x <= 1 else (x - 1).bit_length()
+
+
+def choose(s: int, z: int, b: int = 64) -> int:
+    """Scheme index (0 COO, 1 CSR, 2 bitmap, 3 dense) under the
+    preprocessing/test.cpp:55-77 model."""
+    k, lss = _ceil_log2(s), _ceil_log2(s * s)
+    coo = 2 * k * z + lss
+    csr = k * z + (s + 1) * lss
+    best, sc = coo, 0
+    if csr < best:
+        best, sc = csr, 1
+    if s * s < best:
+        best, sc = s * s, 2
+    if (s * s - z) * b < best:
+        sc = 3
+    return sc
+
+
+def layout_bytes(s, m, n, nb, ne, val_bytes=8):
+    """Stored bytes of the HBM layout for per-scheme block/value counts."""
+    M, N = -(-m // s), -(-n // s)
+    bcol = 2 if N <= 65536 else 4
+    idx = 1 if s <= 16 else 2
+    rp = 1 if s <= 8 else 2
+    W = max(1, s * s // 64)
+    tot = 4 * 2 * 4 * (M + 1)
+    for x in range(4):
+        tot += nb[x] * (bcol + 2) + ne[x] * val_bytes
+    tot += ne[0] * idx + nb[1] * (s + 1) * rp + ne[1] + nb[2] * W * 8
+    return tot
+
+
+# ---- features -----------------------------------------------------------------
+
+def block_hist(rowptr, col, m, n, s, stride, on_device=True, nnz=None, stream=None):
+    """GPU block histogram of s x s blocks over every stride-th strip."""
+    cap = s * s + 2
+    hist = np.zeros(cap, dtype=np.uint64)
+    nbk, ns = C.c_uint64(0), C.c_uint64(0)
+    k = s.bit_length() - 1
+    if on_device:
+        rp_p, c_p = rowptr.ptr, col.ptr
+    else:
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        rp_p, c_p = rowptr.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p)
+        nnz = int(rowptr[-1])
+    capi.call("abhsf_block_hist", rp_p, c_p, m, n, nnz, int(on_device), k, k, stride,
+              hist.ctypes.data_as(C.c_void_p), cap, C.byref(nbk), C.byref(ns),
+              None if stream is None else stream.handle)
+    return hist, int(nbk.value), int(ns.value)
+
+
+def global_features(rowptr, m, n):
+    deg = np.diff(rowptr).astype(np.float64)
+    nnz = float(rowptr[-1])
+    mean = nnz / m
+    sd = float(np.std(deg))
+    return {
+        "log2_n": math.log2(max(n, 1)), "log2_nnz": math.log2(max(nnz, 1)),
+        "log2_mean_row": math.log2(max(mean, 1e-9)), "row_cv": sd / max(mean, 1e-9),
+        "log2_max_row": math.log2(max(deg.max(initial=0.0), 1.0) / max(mean, 1e-9)),
+        "empty_rows": float(np.mean(deg == 0)),
+    }
+
+
+def unit_entries(s):
+    return max(2048, 8 * s)
+
+
+def size_features(rowptr, m, n, s, hist, nblk, sampled):
+    nnz = int(rowptr[-1])
+    scale = nnz / max(sampled, 1)
+    nb = [0.0] * 4
+    ne = [0.0] * 4
+    z = np.nonzero(hist)[0]
+    for zz in z:
+        c = float(hist[zz]) * scale
+        sc = choose(s, int(zz))
+        nb[sc] += c
+        ne[sc] += c * (s * s if sc == 3 else int(zz))
+    totb = max(sum(nb), 1.0)
+    tote = max(sum(ne), 1.0)
+    stored = layout_bytes(s, m, n, nb, ne)
+    M = -(-m // s)
+    rp = rowptr[np.minimum(np.arange(0, M + 1) * s, m)]
+    work = np.diff(rp).astype(np.float64)
+    mean = max(float(work.mean()), 1e-9)
+    H = 2 * unit_entries(s)
+    return {
+        "log2_s": math.log2(s), "log2_blk_per_entry": math.log2(max(nblk, 1) / max(sampled, 1)),
+        "blk_frac": [v / totb for v in nb], "ent_frac": [v / tote for v in ne],
+        "log2_bytes_per_entry": math.log2(max(stored, 1) / max(nnz, 1)),
+        "strip_cv": float(work.std()) / mean, "log2_max_strip": math.log2(max(float(work.max()), 1.0) / mean),
+        "heavy_frac": float(work[work > H].sum()) / max(float(work.sum()), 1.0), "log2_M": math.log2(M),
+        "stored_bytes": stored,
+    }
+
+
+def feature_vector(g, sf, variant):
+    return np.array([
+        g["log2_n"], g["log2_nnz"], g["log2_mean_row"], g["row_cv"], g["log2_max_row"], g["empty_rows"],
+        sf["log2_s"], sf["log2_blk_per_entry"], *sf["blk_frac"], *sf["ent_frac"],
+        sf["log2_bytes_per_entry"], sf["strip_cv"], sf["log2_max_strip"], sf["heavy_frac"], sf["log2_M"],
+        1.0 if variant == "balanced" else 0.0,
+    ], dtype=np.float64)
+
+
+NFEAT = 6 + 15
+
+
+def candidates():
+    return [(s, v) for s in S_CANDIDATES for v in VARIANTS]
+
+
+def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
+    """Feature rows for every candidate of matrix A (synth.Csr)."""
+    g = global_features(A.rowptr, A.m, A.n)
+    stride = max(1, A.nnz // SAMPLE_NNZ)
+    rows = []
+    per_s = {}
+    for s in S_CANDIDATES:
+        if d_rowptr is not None:
+            hist, nblk, ns = block_hist(d_rowptr, d_col, A.m, A.n, s, stride, True, A.nnz, stream)
+        else:
+            hist, nblk, ns = block_hist(A.rowptr, A.col, A.m, A.n, s, stride, False)
+        per_s[s] = size_features(A.rowptr, A.m, A.n, s, hist, nblk, ns)
+    for s, v in candidates():
+        rows.append(feature_vector(g, per_s[s], v))
+    return np.array(rows), per_s
+
+
+# ---- exhaustive timing sweep (labels and regret judge) -------------------------
+
+def time_candidate(A, s, variant, d_rp, d_c, d_v, x, y, stream, reps=10):
+    from .matrix import AbhsfMatrix
+    M = AbhsfMatrix(block_size=s, variant=variant, stream=stream)
+    M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+    for _ in range(2):
+        M.spmv_async(1.0, x, 1.0, y)
+    stream.synchronize()
+    times = []
+    for _ in range(3):
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            M.spmv_async(1.0, x, 1.0, y)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1) / reps)
+    M.release()
+    return float(np.median(times))
+
+
+def sweep(A, stream=None):
+    """Time every candidate on A; returns {(s, variant): ms}."""
+    stream = stream or hip.Stream()
+    d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
+    x = hip.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, A.n))
+    y = hip.DeviceArray.zeros(A.m)
+    return {(s, v): time_candidate(A, s, v, d_rp, d_c, d_v, x, y, stream) for s, v in candidates()}
+
+
+def regret(times, pick):
+    """(t_pick - t_min) / t_min * 100 (block_statistics/consistency.cpp:147-149)."""
+    tmin = min(times.values())
+    return (times[pick] - tmin) / tmin * 100.0
+
+
+# ---- the model ---------------------------------------------------------------------
+
+def init_theta(F, seed=0):
+    rng = np.random.default_rng(seed)
+    P = HIDDEN * F + HIDDEN + HIDDEN * HIDDEN + HIDDEN + HIDDEN + 1
+    t = np.zeros(P, dtype=np.float32)
+    o = 0
+    t[o:o + HIDDEN * F] = rng.uniform(-1, 1, HIDDEN * F) * math.sqrt(6.0 / F)
+    o += HIDDEN * F + HIDDEN
+    t[o:o + HIDDEN * HIDDEN] = rng.uniform(-1, 1, HIDDEN * HIDDEN) * math.sqrt(6.0 / HIDDEN)
+    o += HIDDEN * HIDDEN + HIDDEN
+    t[o:o + HIDDEN] = rng.uniform(-1, 1, HIDDEN) * math.sqrt(6.0 / HIDDEN)
+    return t
+
+
+class Model:
+    def __init__(self, theta, mu, sd, ymu=0.0, meta=None):
+        self.theta = np.asarray(theta, dtype=np.float32)
+        self.mu = np.asarray(mu, dtype=np.float64)
+        self.sd = np.asarray(sd, dtype=np.float64)
+        self.ymu = float(ymu)
+        self.meta = meta or {}
+
+    @classmethod
+    def load(cls, path=MODEL_PATH):
+        d = json.load(open(path))
+        return cls(d["theta"], d["mu"], d["sd"], d.get("ymu", 0.0), d.get("meta"))
+
+    def save(self, path=MODEL_PATH):
+        json.dump({"theta": [float(v) for v in self.theta], "mu": list(map(float, self.mu)),
+                   "sd": list(map(float, self.sd)), "ymu": self.ymu, "meta": self.meta},
+                  open(path, "w"))
+
+    def normalize(self, X):
+        return ((np.asarray(X, dtype=np.float64) - self.mu) / self.sd).astype(np.float32)
+
+    def predict(self, X):
+        """log(ms per entry) predictions, computed by the GPU forward kernel."""
+        Xn = self.normalize(X)
+        B, F = Xn.shape
+        dt = hip.DeviceArray.from_numpy(self.theta)
+        dx = hip.DeviceArray.from_numpy(Xn)
+        h1, h2, out = hip.DeviceArray((B, HIDDEN), np.float32), hip.DeviceArray((B, HIDDEN), np.float32), \
+            hip.DeviceArray((B,), np.float32)
+        capi.call("abhsf_mlp_forward", dt.ptr, F, dx.ptr, B, h1.ptr, h2.ptr, out.ptr, None)
+        hip.synchronize()
+        return out.numpy().astype(np.float64) + self.ymu
+
+
+def train(X, y, w=None, steps=3000, lr=3e-3, seed=0, mu=None, sd=None):
+    """Fit the MLP on the GPU (full-batch Adam by abhsf_mlp_train)."""
+    X = np.asarray(X, dtype=np.float64)
+    mu = X.mean(0) if mu is None else mu
+    sd = X.std(0) if sd is None else sd
+    sd = np.where(sd > 1e-9, sd, 1.0)
+    ymu = float(np.mean(y))
+    model = Model(init_theta(X.shape[1], seed), mu, sd, ymu)
+    Xn = model.normalize(X)
+    dt = hip.DeviceArray.from_numpy(model.theta)
+    dx = hip.DeviceArray.from_numpy(Xn)
+    dy = hip.DeviceArray.from_numpy(np.asarray(y, dtype=np.float32) - np.float32(ymu))
+    dw = None if w is None else hip.DeviceArray.from_numpy(np.asarray(w, dtype=np.float32))
+    losses = np.zeros(steps, dtype=np.float32)
+    t0 = time.perf_counter()
+    capi.call("abhsf_mlp_train", dt.ptr, X.shape[1], dx.ptr, dy.ptr, None if dw is None else dw.ptr,
+              X.shape[0], steps, float(lr), losses.ctypes.data_as(C.c_void_p), None)
+    model.theta = dt.numpy()
+    model.meta = {"steps": steps, "lr": lr, "samples": int(X.shape[0]), "final_loss": float(losses[-1]),
+                  "first_loss": float(losses[0]), "train_s": time.perf_counter() - t0}
+    return model, losses
+
+
+def pick(model, Xc):
+    """Candidate with the smallest predicted time, and the predictions."""
+    pred = model.predict(Xc)
+    return candidates()[int(np.argmin(pred))], pred
+
+
+def pick_storage_optimal(per_s):
+    """The reference's exhaustive search: the block size with the fewest
+    stored bits (block_statistics/mmf.h:77-102), default variant."""
+    s = min(per_s, key=lambda k: per_s[k]["stored_bytes"])
+    return (s, "balanced")
+
+
+def pick_block_size(A, candidates=S_CANDIDATES, model_path=MODEL_PATH):
+    """Block size chosen by the trained model (None when no model is present)."""
+    if not os.path.exists(model_path):
+        return None
+    model = Model.load(model_path)
+    Xc, _ = matrix_features(A)
+    cand = globals()["candidates"]()
+    pred = model.predict(Xc)
+    allowed = [i for i, (s, _) in enumerate(cand) if s in candidates]
+    best = min(allowed, key=lambda i: pred[i])
+    return cand[best][0]

@@ -139,6 +139,41 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
 abhsf_status abhsf_export(const abhsf_handle *h, const char *name, void *dst, size_t cap,
                           size_t *bytes);
 
+/* Block statistics on the GPU: hist[z] = number of nonzero r x s blocks
+ * (r = 2^kr, s = 2^ks, 0 <= kr, ks <= 16) holding z entries (z >= hist_cap - 1
+ * lands in hist[hist_cap - 1]); *nblocks = number of nonzero blocks.  Only
+ * every strip_stride-th strip of r rows is counted (1 = all rows, the
+ * reference's statistic); *sampled_nnz receives the entries counted.  CSR as
+ * for abhsf_build_csr.  Synchronous on `stream` (hipStream_t, may be NULL).
+ * Replaces process_block_size_powers (block_statistics/block_stats.cpp:69-100),
+ * which sorts the whole matrix once per block size. */
+abhsf_status abhsf_block_hist(const int64_t *rowptr, const int32_t *col, int64_t m, int64_t n, int64_t nnz,
+                              int32_t on_device, int32_t kr, int32_t ks, int64_t strip_stride, uint64_t *hist,
+                              int64_t hist_cap, uint64_t *nblocks, uint64_t *sampled_nnz, void *stream);
+
+/* ---- Tuner cost model: 2 x 64 hidden ReLU MLP, FP32, on the GPU ----------
+ * No reference counterpart (the reference searches block sizes exhaustively
+ * over storage bits, block_statistics/mmf.h:77-102); these are the learned
+ * tuner's training and inference kernels.  All arrays are device pointers;
+ * theta holds W1[64][nfeat], b1[64], W2[64][64], b2[64], W3[64], b3.
+ * X is nsamples x nfeat row-major (1 <= nfeat <= 64).  H1, H2, D1, D2 are
+ * nsamples x 64 work arrays; out, y, w, D3 have nsamples entries (w may be
+ * NULL for unit weights, wsum = sum of w).  The loss is
+ * sum_b w_b (out_b - y_b)^2 / wsum. */
+int64_t abhsf_mlp_num_params(int32_t nfeat);
+abhsf_status abhsf_mlp_forward(const float *theta, int32_t nfeat, const float *X, int64_t nsamples, float *H1,
+                               float *H2, float *out, void *stream);
+/* grad (num_params) = dLoss/dtheta, summed over samples in sample order. */
+abhsf_status abhsf_mlp_backward(const float *theta, int32_t nfeat, const float *X, int64_t nsamples,
+                                const float *H1, const float *H2, const float *out, const float *y,
+                                const float *w, float wsum, float *D1, float *D2, float *D3, float *grad,
+                                void *stream);
+/* `steps` full-batch Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) updating
+ * theta in place; loss_host (host, `steps` floats, may be NULL) receives the
+ * loss before each step.  Synchronous on stream. */
+abhsf_status abhsf_mlp_train(float *theta, int32_t nfeat, const float *X, const float *y, const float *w,
+                             int64_t nsamples, int64_t steps, float lr, float *loss_host, void *stream);
+
 /* Wait for all work enqueued on the handle's stream. */
 abhsf_status abhsf_synchronize(abhsf_handle *h);
 

@@ -13,7 +13,7 @@ from conftest import ROOT
 
 def declared_symbols():
     src = open(os.path.join(ROOT, "include", "abhsf_c.h")).read()
-    return sorted(set(re.findall(r"^(?:abhsf_status|const char \*)\s*(abhsf_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^(?:abhsf_status|const char \*|int64_t)\s*(abhsf_\w+)\s*\(", src, re.M)))
 
 
 def test_header_declares_expected_entry_points():

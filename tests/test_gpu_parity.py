@@ -251,3 +251,24 @@ def test_device_csr_input_and_determinism(gpu):
     assert relerr(outs[0], ref) <= TOL
     same = all(np.array_equal(outs[0], o) for o in outs[1:])
     print(f"bitwise-identical repeated SpMV: {same}")
+
+
+@pytest.mark.parametrize("variant", ["strip", "balanced"])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("s", [8, 64, 256])
+def test_heavy_strip_variants(gpu, variant, precision, s):
+    """Power-law rows: heavy strips are split into units whose partial sums
+    are combined by the fix-up pass (balanced) or processed whole (strip)."""
+    A = synth.rmat(15, 16, seed=4)
+    print(f"case rmat15 s={s} {precision} {variant}", flush=True)
+    M = AbhsfMatrix(block_size=s, precision=precision, variant=variant)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    check_arrays(M, A, s, val_bytes=8 if precision == "f64" else 4)
+    x = np.random.default_rng(6).uniform(-1, 1, A.n)
+    y0 = np.random.default_rng(7).uniform(-1, 1, A.m)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y0)
+    M.spmv_async(2.0, dx, 0.5, dy)
+    M.synchronize()
+    vals = A.val if precision == "f64" else A.val.astype(np.float32).astype(np.float64)
+    ref = pyoracle.csr_spmv(A.rowptr, A.col, vals, x, 2.0, 0.5, y0)
+    assert relerr(dy.numpy(), ref) <= TOL

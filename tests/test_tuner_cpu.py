@@ -1,0 +1,67 @@
+"""CPU checks of the tuner's oracle (oracle/mlp_ref.py) and of the tuner's
+storage-cost model against the oracle builder."""
+import numpy as np
+import pytest
+
+from abhsf_amd import synth, tuner
+from oracle import mlp_ref, pyoracle
+
+
+def test_mlp_ref_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    F = 7
+    theta = mlp_ref.init(F, seed=1) * 0.5
+    X = rng.standard_normal((13, F))
+    y = rng.standard_normal(13)
+    w = rng.uniform(0.5, 2.0, 13)
+    g = mlp_ref.grad(theta, X, y, w)
+    idx = rng.choice(theta.size, 80, replace=False)
+    h = 1e-6
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (mlp_ref.loss(tp, X, y, w) - mlp_ref.loss(tm, X, y, w)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), i
+
+
+def test_mlp_ref_closed_form():
+    """W1 = identity on the first F hidden units, W2 = identity, W3 = ones:
+    out = sum(relu(x)) + b3."""
+    F = 5
+    L = mlp_ref.layout(F)
+    t = np.zeros(L["P"])
+    W1 = np.zeros((64, F))
+    W1[:F, :F] = np.eye(F)
+    t[L["W1"][0]:L["W1"][1]] = W1.ravel()
+    t[L["W2"][0]:L["W2"][1]] = np.eye(64).ravel()
+    t[L["W3"][0]:L["W3"][1]] = 1.0
+    t[L["b3"][0]] = 0.25
+    X = np.array([[1.0, -2.0, 3.0, -4.0, 5.0], [-1.0, -1.0, -1.0, -1.0, -1.0]])
+    _, _, out = mlp_ref.forward(t, X)
+    assert np.allclose(out, [9.25, 0.25])
+
+
+@pytest.mark.parametrize("s", [2, 8, 16, 64, 256])
+def test_tuner_choose_matches_oracle(s):
+    for z in range(1, s * s + 1, max(1, s * s // 500)):
+        assert tuner.choose(s, z) == pyoracle.choose(pyoracle.PREPROC, s, z)[0]
+
+
+@pytest.mark.parametrize("s", [8, 32, 64, 128])
+def test_layout_bytes_matches_oracle_storage(s):
+    for A in (synth.block_mix(4096, seed=3), synth.rmat(11, 8, seed=2), synth.laplacian2d(40)):
+        B = pyoracle.Abhsf(A.m, A.n, A.rowptr, A.col, A.val, s.bit_length() - 1)
+        arrays = B.arrays()
+        stored = sum(a.nbytes for a in arrays.values())
+        assert tuner.layout_bytes(s, A.m, A.n, B.nb, [int(v) for v in B.st.ne]) == stored
+        # features from an exact histogram reproduce the oracle's block counts
+        r, c, _ = A.triplets()
+        h = pyoracle.block_hist(r, c, s.bit_length() - 1, s.bit_length() - 1)
+        hist = np.zeros(s * s + 2, dtype=np.uint64)
+        for z, cnt in h.items():
+            hist[z] = cnt
+        sf = tuner.size_features(A.rowptr, A.m, A.n, s, hist, sum(h.values()), A.nnz)
+        assert sf["stored_bytes"] == pytest.approx(stored)
+        tot = sum(B.nb)
+        assert np.allclose(sf["blk_frac"], [v / tot for v in B.nb])
