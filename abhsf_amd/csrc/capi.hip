@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -27,6 +28,19 @@ struct abhsf_handle {
     abhsf_options opts;
     hipStream_t stream;
     abhsf::Storage S;
+};
+
+struct abhsf_batch {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<abhsf::BatchEntry> host;
+    abhsf::DevBuf table, heavy;
+    int64_t total_items = 0, nheavy = 0;
+    int grid = 1;
+    bool vectors_set = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double galpha = 0.0, gbeta = 0.0;
 };
 
 namespace {
@@ -302,6 +316,118 @@ abhsf_status abhsf_mlp_train(float *theta, int32_t nfeat, const float *X, const 
     if (st != ABHSF_OK) return fail(st, err);
     g_last_error.clear();
     return ABHSF_OK;
+}
+
+abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *stream, abhsf_batch **out)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(mats && out && count >= 1, ABHSF_ERR_INVALID, "invalid argument");
+        *out = nullptr;
+        auto b = std::make_unique<abhsf_batch>();
+        if (stream) {
+            b->stream = static_cast<hipStream_t>(stream);
+        } else {
+            ABHSF_HIP(hipStreamCreate(&b->stream));
+            b->own_stream = true;
+        }
+        std::vector<uint32_t> hv;
+        int64_t item = 0;
+        for (int32_t i = 0; i < count; i++) {
+            const abhsf_handle *h = mats[i];
+            ABHSF_REQUIRE(h && h->S.built, ABHSF_ERR_STATE, "batch member not built");
+            ABHSF_REQUIRE(h->S.val_bytes == 8, ABHSF_ERR_UNSUPPORTED, "batched launch needs FP64 storage");
+            abhsf::BatchEntry e{};
+            e.A = h->S.view();
+            e.B = h->S.launch().bal;
+            if (!e.B.light && e.B.n_units == 0) e.B.n_light = h->S.M;
+            e.item0 = item;
+            e.code = abhsf::batch_code(h->S.bcol_bytes, h->S.s);
+            item += e.B.n_light + e.B.n_units;
+            for (int64_t q = 0; q < e.B.n_heavy; q++) hv.insert(hv.end(), {(uint32_t)i, (uint32_t)q});
+            b->host.push_back(e);
+        }
+        b->total_items = item;
+        b->nheavy = (int64_t)hv.size() / 2;
+        const int64_t want = (item + 3) / 4;
+        b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 8192));
+        b->table.alloc(b->host.size() * sizeof(abhsf::BatchEntry));
+        b->heavy.alloc(hv.size() * 4);
+        if (!hv.empty())
+            ABHSF_HIP(hipMemcpyAsync(b->heavy.get(), hv.data(), hv.size() * 4, hipMemcpyHostToDevice, b->stream));
+        ABHSF_HIP(hipStreamSynchronize(b->stream));
+        *out = b.release();
+    });
+}
+
+abhsf_status abhsf_batch_set_vectors(abhsf_batch *b, const double *const *xs, double *const *ys)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(b && xs && ys, ABHSF_ERR_INVALID, "null argument");
+        for (size_t i = 0; i < b->host.size(); i++) {
+            ABHSF_REQUIRE(xs[i] && ys[i], ABHSF_ERR_INVALID, "null vector in batch");
+            b->host[i].x = xs[i];
+            b->host[i].y = ys[i];
+        }
+        ABHSF_HIP(hipMemcpyAsync(b->table.get(), b->host.data(), b->table.bytes(), hipMemcpyHostToDevice, b->stream));
+        ABHSF_HIP(hipStreamSynchronize(b->stream));
+        b->vectors_set = true;
+    });
+}
+
+abhsf_status abhsf_batch_spmv(abhsf_batch *b, double alpha, double beta, int32_t use_graph)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(b, ABHSF_ERR_INVALID, "null batch");
+        ABHSF_REQUIRE(b->vectors_set, ABHSF_ERR_STATE, "abhsf_batch_set_vectors not called");
+        auto launch = [&] {
+            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), (int)b->host.size(), b->total_items,
+                              b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->grid, b->stream);
+        };
+        if (!use_graph) {
+            launch();
+            return;
+        }
+        if (!b->exec || b->galpha != alpha || b->gbeta != beta) {
+            if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+            ABHSF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(b->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            ABHSF_HIP(hipStreamEndCapture(b->stream, &b->graph));
+            ABHSF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->galpha = alpha;
+            b->gbeta = beta;
+        }
+        ABHSF_HIP(hipGraphLaunch(b->exec, b->stream));
+    });
+}
+
+abhsf_status abhsf_batch_synchronize(abhsf_batch *b)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(b, ABHSF_ERR_INVALID, "null batch");
+        ABHSF_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+abhsf_status abhsf_batch_destroy(abhsf_batch *b)
+{
+    return guarded([&] {
+        if (!b) return;
+        (void)hipStreamSynchronize(b->stream);
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->own_stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+    });
 }
 
 abhsf_status abhsf_synchronize(abhsf_handle *h)

@@ -365,22 +365,17 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
 // itself when light is null) whose rows are written to y; items n_light + u are
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
+// One work item of one matrix (see Balance): the wave accumulates in acc_buf
+// (s doubles, 2s when compensated) and writes y rows or a unit's partials.
 template <typename VT, typename BT, typename IT, typename RT, bool COMP>
-__global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__restrict__ x, double *__restrict__ y,
-                                                    double alpha, double beta, Balance B)
+__device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item, const double *__restrict__ x,
+                                 double *__restrict__ y, double alpha, double beta, double *acc_buf,
+                                 uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, int lane)
 {
-    constexpr int kWaves = 4;
-    __shared__ double s_acc[kWaves][256 * (COMP ? 2 : 1)];
-    __shared__ uint32_t s_end[kWaves][kWave];
-    __shared__ uint32_t s_bc[kWaves][kWave];
-    __shared__ uint32_t s_st[kWaves][kWave];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t item = (int64_t)blockIdx.x * kWaves + wave;
-    if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
     const int s = A.s;
     Acc acc;
-    acc.hi = s_acc[wave];
-    acc.lo = COMP ? s_acc[wave] + 256 : nullptr;
+    acc.hi = acc_buf;
+    acc.lo = COMP ? acc_buf + 256 : nullptr;
     for (int i = lane; i < s; i += kWave) {
         acc.hi[i] = 0.0;
         if constexpr (COMP) acc.lo[i] = 0.0;
@@ -388,10 +383,10 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     wave_sync();
     if (item < B.n_light) {
         const int64_t I = B.light ? (int64_t)B.light[item] : item;
-        coo_r1<VT, BT, IT, COMP, 2>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end[wave],
-                                    s_bc[wave], s_st[wave], lane);
-        coo_csr_r1<VT, BT, IT, RT, COMP>(A, CSR, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc,
-                                         s_end[wave], s_bc[wave], lane);
+        coo_r1<VT, BT, IT, COMP, 2>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc, s_st,
+                                    lane);
+        coo_csr_r1<VT, BT, IT, RT, COMP>(A, CSR, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc, s_end,
+                                         s_bc, lane);
         wave_sync();
         bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x, acc, lane);
         bmdn_r1<VT, BT, COMP>(A, DENSE, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], 0, x, acc, lane);
@@ -408,9 +403,9 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
         const uint32_t *U = B.units + u * 4;
         const uint32_t sc = U[0] & 3u, b0 = U[1], b1 = U[2], e0 = U[3];
         if (sc == COO)
-            coo_r1<VT, BT, IT, COMP, 2>(A, b0, b1, e0, x, acc, s_end[wave], s_bc[wave], s_st[wave], lane);
+            coo_r1<VT, BT, IT, COMP, 2>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, lane);
         else if (sc == CSR)
-            coo_csr_r1<VT, BT, IT, RT, COMP>(A, (int)sc, b0, b1, e0, x, acc, s_end[wave], s_bc[wave], lane);
+            coo_csr_r1<VT, BT, IT, RT, COMP>(A, (int)sc, b0, b1, e0, x, acc, s_end, s_bc, lane);
         else
             bmdn_r1<VT, BT, COMP>(A, (int)sc, b0, b1, e0, x, acc, lane);
         wave_sync();
@@ -419,6 +414,81 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
             part[r] = acc.hi[r];
             if constexpr (COMP) part[s + r] = acc.lo[r];
         }
+    }
+    wave_sync();  // the next item reuses the LDS buffers
+}
+
+// Work items: items [0, n_light) are whole strips (light[item], or the item
+// itself when light is null) whose rows are written to y; items n_light + u are
+// units u of heavy strips: one scheme group's block range, whose partial sums
+// go to part[u] and are combined by k_spmv_fixup.
+template <typename VT, typename BT, typename IT, typename RT, bool COMP>
+__global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__restrict__ x, double *__restrict__ y,
+                                                    double alpha, double beta, Balance B)
+{
+    constexpr int kWaves = 4;
+    __shared__ double s_acc[kWaves][256 * (COMP ? 2 : 1)];
+    __shared__ uint32_t s_end[kWaves][kWave];
+    __shared__ uint32_t s_bc[kWaves][kWave];
+    __shared__ uint32_t s_st[kWaves][kWave];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * kWaves + wave;
+    if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
+    spmv_item<VT, BT, IT, RT, COMP>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave], s_bc[wave], s_st[wave],
+                                    lane);
+}
+
+// Many matrices in one launch: wave w takes global items w, w + W, ... (W =
+// all waves of the grid); the matrix of an item is found by binary search over
+// the entries' first-item offsets; the block-index types are dispatched per
+// matrix.  FP64 values.
+__global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict__ E, int nmat, int64_t total,
+                                                    double alpha, double beta)
+{
+    constexpr int kWaves = 4;
+    __shared__ double s_acc[kWaves][256];
+    __shared__ uint32_t s_end[kWaves][kWave];
+    __shared__ uint32_t s_bc[kWaves][kWave];
+    __shared__ uint32_t s_st[kWaves][kWave];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * kWaves;
+#pragma unroll 1
+    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < total; g += nw) {
+        int lo = 0, hi = nmat - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (E[mid].item0 <= g) lo = mid; else hi = mid - 1;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        const BatchEntry &e = E[lo];
+        const int64_t it = g - e.item0;
+        double *acc = s_acc[wave];
+        uint32_t *se = s_end[wave], *sb = s_bc[wave], *ss = s_st[wave];
+        switch (e.code) {
+        case 0: spmv_item<double, uint16_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
+        case 1: spmv_item<double, uint16_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
+        case 2: spmv_item<double, uint16_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
+        case 3: spmv_item<double, uint32_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
+        case 4: spmv_item<double, uint32_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
+        default: spmv_item<double, uint32_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
+        }
+    }
+}
+
+// Heavy strips of every matrix: hv[2h] = matrix, hv[2h+1] = heavy index.
+__global__ __launch_bounds__(64) void k_batch_fixup(const BatchEntry *__restrict__ E, const uint32_t *__restrict__ hv,
+                                                    double alpha, double beta)
+{
+    const BatchEntry &e = E[hv[2 * blockIdx.x]];
+    const uint32_t h = hv[2 * blockIdx.x + 1];
+    const int s = e.A.s;
+    const uint32_t I = e.B.heavy[h * 3], u0 = e.B.heavy[h * 3 + 1], nu = e.B.heavy[h * 3 + 2];
+    for (int r = threadIdx.x; r < s; r += blockDim.x) {
+        const int64_t gr = (int64_t)I * s + r;
+        if (gr >= e.A.m) break;
+        double a = 0.0;
+        for (uint32_t u = u0; u < u0 + nu; u++) a += e.B.part[(size_t)u * s + r];
+        e.y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * e.y[gr];
     }
 }
 
@@ -452,6 +522,39 @@ __global__ __launch_bounds__(256) void k_spmv_fixup(int s, int64_t m, double *__
 // ===========================================================================
 // Several right-hand sides (SpMM), R lanes per entry.
 // ===========================================================================
+// Dense s x s block times an s x R slice of X on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64: A 16x4 with lane l holding A[l%16][l/16], B 4x16
+// with lane l holding B[l/16][l%16], D 16x16 with lane l holding
+// D[4*(l/16)+r][l%16], r = 0..3).  The 16 x 16 product tiles are added to the
+// strip accumulator (one lane per element, so no atomics).
+template <typename VT, int R>
+__device__ inline void dense_block_mfma(const VT *__restrict__ D, int s, int64_t c0, int64_t n,
+                                        const double *__restrict__ X, int64_t ldx, int nrhs, double *acc, int lane)
+{
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 1
+    for (int r0 = 0; r0 < s; r0 += 16) {
+#pragma unroll
+        for (int j0 = 0; j0 < R; j0 += 16) {
+            typedef double v4d __attribute__((ext_vector_type(4)));
+            v4d c = {0.0, 0.0, 0.0, 0.0};
+            const int j = j0 + li;
+#pragma unroll 4
+            for (int k0 = 0; k0 < s; k0 += 4) {
+                const double a = (double)D[(int64_t)(r0 + li) * s + k0 + lk];
+                const int64_t gc = c0 + k0 + lk;
+                const double bx = (gc < n && j < nrhs) ? X[gc * ldx + j] : 0.0;
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bx, c, 0, 0, 0);
+            }
+            const int rb = r0 + 4 * lk;
+            acc[(rb + 0) * R + j] += c[0];
+            acc[(rb + 1) * R + j] += c[1];
+            acc[(rb + 2) * R + j] += c[2];
+            acc[(rb + 3) * R + j] += c[3];
+        }
+    }
+}
+
 template <typename VT, typename BT, typename IT, typename RT, int R, bool COMP>
 __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__restrict__ X, int64_t ldx,
                                                     double *__restrict__ Y, int64_t ldy, int nrhs,
@@ -556,6 +659,12 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
             const int64_t c0 = (int64_t)ld<BT>(A.bcol[sc], b) * s;
             const uint64_t *words = A.bm_words + (uint64_t)b * W;
             if (sc == DENSE) vbase = (uint64_t)b * ssq;
+            if constexpr (R >= 16 && !COMP) {
+                if (sc == DENSE && s >= 16) {
+                    dense_block_mfma<VT, R>(val + vbase, s, c0, A.n, X, ldx, nrhs, acc.hi, lane);
+                    continue;
+                }
+            }
 #pragma unroll 1
             for (int w = 0; w < W; w++) {
                 const uint64_t word = (sc == BITMAP) ? words[w] : ~0ull;
@@ -691,6 +800,21 @@ void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int6
 }
 
 }  // namespace
+
+int batch_code(int bcol_bytes, int s) { return (bcol_bytes == 4 ? 3 : 0) + (s <= 8 ? 0 : (s <= 16 ? 1 : 2)); }
+
+void batch_spmv(const BatchEntry *dE, int nmat, int64_t total_items, const uint32_t *dheavy, int64_t nheavy,
+                double alpha, double beta, int grid, hipStream_t st)
+{
+    if (total_items > 0) {
+        hipLaunchKernelGGL(k_batch_spmv, dim3((unsigned)grid), dim3(256), 0, st, dE, nmat, total_items, alpha, beta);
+        ABHSF_HIP(hipGetLastError());
+    }
+    if (nheavy > 0) {
+        hipLaunchKernelGGL(k_batch_fixup, dim3((unsigned)nheavy), dim3(64), 0, st, dE, dheavy, alpha, beta);
+        ABHSF_HIP(hipGetLastError());
+    }
+}
 
 int max_rhs_tile(int s, bool comp)
 {

@@ -22,6 +22,23 @@ struct SpmvLaunch {
     Balance bal;
 };
 
+// One matrix of a batched launch (device-resident table entry).
+struct BatchEntry {
+    DevMat A;
+    Balance B;
+    const double *x;
+    double *y;
+    int64_t item0;  // first global work item of this matrix
+    int32_t code;   // index-type combination, batch_code()
+};
+
+int batch_code(int bcol_bytes, int s);
+
+// All matrices' work items in one grid-stride launch, then the heavy-strip
+// fix-up (dheavy: pairs {matrix, heavy index}).  FP64 storage only.
+void batch_spmv(const BatchEntry *dE, int nmat, int64_t total_items, const uint32_t *dheavy, int64_t nheavy,
+                double alpha, double beta, int grid, hipStream_t st);
+
 // Largest power-of-two rhs tile whose s x R accumulator fits the LDS budget.
 int max_rhs_tile(int s, bool comp);
 

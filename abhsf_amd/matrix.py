@@ -162,3 +162,41 @@ class AbhsfMatrix:
             self.release()
         except Exception:
             pass
+
+
+class AbhsfBatch:
+    """Many built matrices driven by one launch (optionally a replayed hipGraph).
+
+    The members must stay alive and unchanged while the batch exists.
+    """
+
+    def __init__(self, mats, stream=None):
+        self.mats = list(mats)
+        arr = (C.c_void_p * len(self.mats))(*[m._h.value for m in self.mats])
+        self._h = C.c_void_p(0)
+        capi.call("abhsf_batch_create", arr, len(self.mats),
+                  stream.handle if isinstance(stream, hip.Stream) else stream, C.byref(self._h))
+        self._vec = None
+
+    def set_vectors(self, xs, ys) -> None:
+        xa = (C.c_void_p * len(xs))(*[_addr(v) for v in xs])
+        ya = (C.c_void_p * len(ys))(*[_addr(v) for v in ys])
+        capi.call("abhsf_batch_set_vectors", self._h, xa, ya)
+        self._vec = (xs, ys)  # keep the device vectors alive
+
+    def spmv_async(self, alpha: float = 1.0, beta: float = 0.0, graph: bool = False) -> None:
+        capi.call("abhsf_batch_spmv", self._h, float(alpha), float(beta), int(graph))
+
+    def synchronize(self) -> None:
+        capi.call("abhsf_batch_synchronize", self._h)
+
+    def release(self) -> None:
+        if self._h and self._h.value:
+            capi.call("abhsf_batch_destroy", self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

@@ -1,26 +1,30 @@
 """ABHSF SpMV benchmark on one MI355X.
 
-Workload (the largest single-GPU configuration the metric is quoted on):
-block-structured synthetic matrix, n = 2^22, 32-row strips drawn 20% three
-dense 32x32 blocks / 30% block-tridiagonal of 8x8 sub-blocks / 50% 12 uniform
-columns per row, values N(0,1) FP64 (seed 0), x ~ U[-1,1] (seed 0); ABHSF at
-s in {16, 32, 64, 128}.  A step is one SpMV y += A*x (the reference harness's
-accumulating call, spmv/unified/spmv.cpp:1222-1231) with all inputs resident
-in HBM.
+Default workload (the largest single-GPU configuration the metric is quoted
+on): block-structured synthetic matrix, n = 2^22, 32-row strips drawn 20%
+three dense 32x32 blocks / 30% block-tridiagonal of 8x8 sub-blocks / 50% 12
+uniform columns per row, values N(0,1) FP64 (seed 0), x ~ U[-1,1] (seed 0);
+ABHSF at s in {16, 32, 64, 128}.  A step is one SpMV y += A*x (the reference
+harness's accumulating call, spmv/unified/spmv.cpp:1222-1231) with all inputs
+resident in HBM.
 
-value = GFLOP/s (2*nnz per step) of the block size chosen by the tuner (the
-cost-model tuner picks s and kernel variant from matrix statistics); every
-measured s is reported, as is plain ABHSF at fixed s = 64.  The roofline entry
-is for the SpMV kernel at the chosen s: algorithmic bytes per launch = all
-ABHSF arrays as stored + 8n (x) + 16m (y read and written), divided by the
-kernel's average duration from HIP events on the handle's stream.
+value = GFLOP/s (2*nnz per step) at the (s, variant) the learned tuner picks
+(abhsf_amd/tuner.py, MLP on GPU block statistics); every measured s is
+reported, and plain ABHSF at fixed s = 64.  The roofline entry is for the SpMV
+kernel at the chosen s: algorithmic bytes per launch = all ABHSF arrays as
+stored + 8n (x) + 16m (y read and written), divided by the kernel's average
+duration from HIP events on the handle's stream.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--s 16,32,64,128]
-                       [--n 4194304] [--no-cpu-baseline] [--no-check]
+Other workloads (--workload): rmat (R-MAT scale 22, edge factor 16; tuned vs
+plain with exhaustive regret), spmm (27-point stencil 160^3, k in
+{1,4,8,16,32}, FP64 and FP32 storage), batch (2000 matrices in one launch).
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload block_mix]
 """
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -31,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "SpMV GFLOP/s + % of HBM roofline; learned-tuned vs plain ABHSF, same GPU"
 
 
 def log(msg):
@@ -53,7 +58,7 @@ def calib_copy_gbs(hip, nbytes=2 << 30, reps=10):
     e1.record(st)
     e1.synchronize()
     ms = e0.elapsed_ms(e1) / reps
-    return 2 * nbytes / (ms * 1e-3) / 1e9, ms
+    return {"copy_gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1), "bytes": 2 * nbytes, "ms": round(ms, 3)}
 
 
 def time_spmv(M, hip, dx, dy, steps, warmup, stream):
@@ -87,12 +92,249 @@ def cpu_baseline(A, s, x, seconds_target=10.0):
         el = time.perf_counter() - t0
         if el >= seconds_target or it >= 200:
             break
-    gflops = 2.0 * nnz * it / el / 1e9
     return {
-        "value": round(gflops, 3), "unit": "GFLOP/s", "cores": int(pyoracle.lib().orc_num_threads()),
-        "kind": "port",
+        "value": round(2.0 * nnz * it / el / 1e9, 3), "unit": "GFLOP/s",
+        "cores": int(pyoracle.lib().orc_num_threads()), "kind": "port",
         "sample": f"first {rows} rows ({nnz} nnz) of the same matrix, s={s}, {it} SpMV calls in {el:.1f} s",
     }
+
+
+def measure_candidates(A, cands, args, hip, stream, check=True):
+    """Build and time each (s, variant); returns per-candidate results."""
+    from abhsf_amd import AbhsfMatrix
+    x = np.random.default_rng(0).uniform(-1.0, 1.0, A.n)
+    dx = hip.DeviceArray.from_numpy(x)
+    dy = hip.DeviceArray.zeros(A.m)
+    d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
+    ref = None
+    if check:
+        from oracle import pyoracle
+        ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)
+    res = {}
+    for s, v in cands:
+        M = AbhsfMatrix(block_size=s, variant=v, stream=stream)
+        M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+        st = M.stats()
+        err = None
+        if ref is not None:
+            dy.fill_zero(stream.handle)
+            M.spmv_async(1.0, dx, 0.0, dy)
+            stream.synchronize()
+            err = float(np.max(np.abs(dy.numpy() - ref)) / np.max(np.abs(ref)))
+            if err > 1e-12:
+                raise SystemExit(f"parity failure at s={s} {v}: rel err {err}")
+        dy.fill_zero(stream.handle)
+        ms = time_spmv(M, hip, dx, dy, args.steps, args.warmup, stream) / args.steps
+        algo = st["stored_bytes"] + 8 * A.n + 16 * A.m
+        res[(s, v)] = {
+            "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2), "ms": round(ms, 4),
+            "gbs": round(algo / (ms * 1e-3) / 1e9, 1), "stored_bytes": st["stored_bytes"],
+            "index_bytes": st["index_bytes"], "algo_bytes": algo, "blocks": st["blocks"],
+            "build_ms": round(st["build_ms"], 1), "rel_err": err,
+        }
+        log(f"s={s} {v}: {res[(s, v)]['gflops']} GFLOP/s, {ms:.4f} ms/SpMV, {res[(s, v)]['gbs']} GB/s, "
+            f"blocks={st['blocks']}, build {st['build_ms']:.0f} ms, err={err}")
+        M.release()
+    return res, x
+
+
+def tuned_pick(A, allowed_s, hip):
+    from abhsf_amd import tuner
+    if not os.path.exists(tuner.MODEL_PATH):
+        return None, "none"
+    model = tuner.Model.load()
+    Xc, _ = tuner.matrix_features(A)
+    pred = model.predict(Xc)
+    cands = tuner.candidates()
+    ok = [i for i, (s, _) in enumerate(cands) if s in allowed_s]
+    best = min(ok, key=lambda i: pred[i])
+    return cands[best], tuner.NAME
+
+
+def roofline(r):
+    return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None}
+
+
+def base_record(args, value, ms, dtype="f64"):
+    return {"metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic"}
+
+
+def run_block_mix(args, hip):
+    from abhsf_amd import synth
+    t0 = time.perf_counter()
+    A = synth.block_mix(args.n, seed=0)
+    log(f"matrix block_mix n={A.n} nnz={A.nnz} checksum={A.checksum():#x} ({time.perf_counter() - t0:.1f} s)")
+    stream = hip.Stream()
+    s_list = [int(v) for v in args.s.split(",")]
+    pick, tname = tuned_pick(A, s_list, hip)
+    cands = [(s, "balanced") for s in s_list]
+    if pick is not None and pick not in cands:
+        cands.append(pick)
+    res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
+    if pick is None:  # no trained model: exhaustive timing decides
+        pick = max(res, key=lambda k: res[k]["gflops"])
+        tname = "exhaustive-timing"
+    best = res[pick]
+    plain = res.get((64, "balanced"))
+    out = base_record(args, best["gflops"], best["ms"])
+    out["config"] = {
+        "workload": f"block_mix n=2^{args.n.bit_length() - 1} (20% 3x dense 32x32, 30% tridiag 8x8, "
+                    f"50% 12/row), N(0,1) FP64 values, seed 0",
+        "nnz": A.nnz, "s": pick[0], "variant": pick[1], "s_measured": s_list, "tuner": tname,
+    }
+    out["roofline"] = roofline(best)
+    out["plain_s64"] = None if plain is None else {"gflops": plain["gflops"], "gbs": plain["gbs"]}
+    out["tuned_vs_plain"] = None if plain is None else round(best["gflops"] / plain["gflops"], 3)
+    out["exhaustive_best"] = {"s": max(res, key=lambda k: res[k]["gflops"])[0],
+                              "gflops": max(r["gflops"] for r in res.values())}
+    out["per_s"] = {f"{s}/{v}": r for (s, v), r in res.items()}
+    if not args.no_calib:
+        out["copy_calibration"] = calib_copy_gbs(hip)
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(A, pick[0], x)
+    return out
+
+
+def run_rmat(args, hip):
+    from abhsf_amd import synth, tuner
+    A = synth.rmat(22, 16, seed=0)
+    log(f"matrix rmat scale 22 ef 16 nnz={A.nnz}")
+    stream = hip.Stream()
+    pick, tname = tuned_pick(A, tuner.S_CANDIDATES, hip)
+    cands = tuner.candidates()
+    res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
+    if pick is None:
+        pick, tname = max(res, key=lambda k: res[k]["gflops"]), "exhaustive-timing"
+    best, plain = res[pick], res[(64, "balanced")]
+    times = {k: r["ms"] for k, r in res.items()}
+    out = base_record(args, best["gflops"], best["ms"])
+    out["config"] = {"workload": "R-MAT scale 22 (a=.57,b=.19,c=.19), edge factor 16, duplicates merged, "
+                     "U[-1,1] FP64 values, seed 0", "nnz": A.nnz, "s": pick[0], "variant": pick[1],
+                     "tuner": tname}
+    out["roofline"] = roofline(best)
+    out["plain_s64"] = {"gflops": plain["gflops"], "gbs": plain["gbs"]}
+    out["tuned_vs_plain"] = round(best["gflops"] / plain["gflops"], 3)
+    out["regret_pct"] = round(tuner.regret(times, pick), 2)
+    out["per_candidate"] = {f"{s}/{v}": r for (s, v), r in res.items()}
+    return out
+
+
+def run_spmm(args, hip):
+    from abhsf_amd import AbhsfMatrix, synth
+    from oracle import pyoracle
+    A = synth.stencil27(160)
+    log(f"matrix stencil27 160^3 nnz={A.nnz}")
+    stream = hip.Stream()
+    s = args.spmm_s
+    d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
+    out_k = {}
+    for prec in ("f64", "f32"):
+        M = AbhsfMatrix(block_size=s, precision=prec, stream=stream)
+        M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+        st = M.stats()
+        for k in (1, 4, 8, 16, 32):
+            X = np.random.default_rng(k).uniform(-1, 1, (A.n, k))
+            dX, dY = hip.DeviceArray.from_numpy(X), hip.DeviceArray.zeros((A.m, k))
+            for _ in range(args.warmup):
+                M.spmm_async(k, 1.0, dX, k, 1.0, dY, k)
+            stream.synchronize()
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            for _ in range(args.steps):
+                M.spmm_async(k, 1.0, dX, k, 1.0, dY, k)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / args.steps
+            # error of Y = A X against the FP64 oracle (columns checked: all for k <= 8, else 8)
+            M.spmm_async(k, 1.0, dX, k, 0.0, dY, k)
+            stream.synchronize()
+            Y = dY.numpy()
+            cols = range(k) if k <= 8 else range(0, k, k // 8)
+            err = 0.0
+            for j in cols:
+                ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, X[:, j].copy())
+                err = max(err, float(np.max(np.abs(Y[:, j] - ref)) / np.max(np.abs(ref))))
+            algo = st["stored_bytes"] + 8 * A.n * k + 16 * A.m * k
+            out_k[f"{prec}/k{k}"] = {"gflops": round(2 * A.nnz * k / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 4),
+                                     "gbs": round(algo / (ms * 1e-3) / 1e9, 1), "maxnorm_rel_err": err}
+            log(f"{prec} k={k}: {out_k[f'{prec}/k{k}']}")
+        M.release()
+    best = max(out_k.values(), key=lambda r: r["gflops"])
+    out = base_record(args, best["gflops"], best["ms"])
+    out["config"] = {"workload": f"27-point stencil 160^3 (nnz {A.nnz}), k in 1,4,8,16,32, s={s}",
+                     "nnz": A.nnz}
+    out["roofline"] = roofline(out_k["f64/k32"])
+    out["per_k"] = out_k
+    return out
+
+
+def run_batch(args, hip):
+    from abhsf_amd import AbhsfBatch, AbhsfMatrix, synth, tuner
+    rng = np.random.default_rng(0)
+    stream = hip.Stream()
+    model = tuner.Model.load() if os.path.exists(tuner.MODEL_PATH) else None
+    mats, xs, ys, nnz, indiv = [], [], [], 0, []
+    t0 = time.perf_counter()
+    for i in range(args.batch_count):
+        n = int(math.exp(rng.uniform(math.log(1e3), math.log(2e5))))
+        fam = int(rng.integers(0, 4))
+        if fam == 0:
+            A = synth.laplacian2d(max(2, int(round(math.sqrt(n)))))
+        elif fam == 1:
+            A = synth.banded(n, int(rng.integers(4, 65)), seed=i)
+        elif fam == 2:
+            A = synth.rmat(max(4, int(round(math.log2(n)))), 8, seed=i)
+        else:
+            A = synth.random_blocks(n, int(rng.choice([4, 8, 16, 32])), int(rng.integers(2, 9)),
+                                    float(rng.uniform(0.01, 0.30)), seed=i)
+        s, v = 64, "balanced"
+        if model is not None:
+            (s, v), _ = tuner.pick(model, tuner.matrix_features(A)[0])
+        M = AbhsfMatrix(block_size=s, variant=v, stream=stream)
+        M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+        x = hip.DeviceArray.from_numpy(np.random.default_rng(i).uniform(-1, 1, A.n))
+        y = hip.DeviceArray.zeros(A.m)
+        mats.append(M)
+        xs.append(x)
+        ys.append(y)
+        nnz += A.nnz
+        if i % 200 == 0:
+            log(f"built {i} matrices, {nnz} nnz ({time.perf_counter() - t0:.0f} s)")
+    # per-matrix latency: each matrix alone
+    for M, x, y in zip(mats, xs, ys):
+        M.spmv_async(1.0, x, 1.0, y)
+        stream.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        M.spmv_async(1.0, x, 1.0, y)
+        e1.record(stream)
+        e1.synchronize()
+        indiv.append(e0.elapsed_ms(e1))
+    B = AbhsfBatch(mats, stream)
+    B.set_vectors(xs, ys)
+    for _ in range(args.warmup):
+        B.spmv_async(1.0, 1.0, graph=True)
+    B.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(args.steps):
+        B.spmv_async(1.0, 1.0, graph=True)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    out = base_record(args, round(2 * nnz / (ms * 1e-3) / 1e9, 2), round(ms, 4))
+    out["config"] = {"workload": f"{args.batch_count} matrices, n log-uniform [1e3, 2e5], families Laplacian / "
+                     "banded bw 4-64 / R-MAT ef 8 / random blocks 1-30%, N(0,1) FP64, one hipGraph launch",
+                     "nnz": nnz, "tuner": tuner.NAME if model is not None else "fixed s=64"}
+    out["per_matrix_latency_ms"] = {"p50": round(float(np.percentile(indiv, 50)), 4),
+                                    "p99": round(float(np.percentile(indiv, 99)), 4),
+                                    "sum_individual": round(float(np.sum(indiv)), 3)}
+    out["batched_vs_individual"] = round(float(np.sum(indiv)) / ms, 2)
+    return out
 
 
 def main():
@@ -100,8 +342,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="block_mix", choices=["block_mix", "rmat", "spmm", "batch"])
     ap.add_argument("--s", default="16,32,64,128")
     ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--spmm-s", type=int, default=64)
+    ap.add_argument("--batch-count", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-calib", action="store_true")
@@ -110,99 +355,9 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
               "(the SpMV shards by block-row strips; only the single-GPU kernel is built)", file=sys.stderr)
         sys.exit(2)
-
-    from abhsf_amd import AbhsfMatrix, hip, synth
-    from abhsf_amd import tuner
-
-    t0 = time.perf_counter()
-    A = synth.block_mix(args.n, seed=0)
-    log(f"matrix block_mix n={A.n} nnz={A.nnz} checksum={A.checksum():#x} ({time.perf_counter() - t0:.1f} s)")
-    x = np.random.default_rng(0).uniform(-1.0, 1.0, A.n)
-    stream = hip.Stream()
-    dx = hip.DeviceArray.from_numpy(x)
-    dy = hip.DeviceArray.zeros(A.m)
-    d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
-    ref = None
-    if not args.no_check:
-        from oracle import pyoracle
-        ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)
-
-    s_list = [int(v) for v in args.s.split(",")]
-    per_s = {}
-    for s in s_list:
-        M = AbhsfMatrix(block_size=s, stream=stream)
-        M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
-        st = M.stats()
-        err = None
-        if ref is not None:
-            dy.fill_zero(stream.handle)
-            M.spmv_async(1.0, dx, 0.0, dy)
-            stream.synchronize()
-            got = dy.numpy()
-            err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
-            if err > 1e-12:
-                raise SystemExit(f"parity failure at s={s}: rel err {err}")
-        dy.fill_zero(stream.handle)
-        ms = time_spmv(M, hip, dx, dy, args.steps, args.warmup, stream)
-        t_launch = ms / args.steps
-        algo_bytes = st["stored_bytes"] + 8 * A.n + 16 * A.m
-        gflops = 2.0 * A.nnz / (t_launch * 1e-3) / 1e9
-        per_s[s] = {
-            "gflops": round(gflops, 2), "ms": round(t_launch, 4), "gbs": round(algo_bytes / (t_launch * 1e-3) / 1e9, 1),
-            "stored_bytes": st["stored_bytes"], "index_bytes": st["index_bytes"], "algo_bytes": algo_bytes,
-            "blocks": st["blocks"], "build_ms": round(st["build_ms"], 1), "rel_err": err,
-        }
-        log(f"s={s}: {gflops:.1f} GFLOP/s, {t_launch:.4f} ms/SpMV, {per_s[s]['gbs']} GB/s algorithmic, "
-            f"blocks={st['blocks']}, build {st['build_ms']:.0f} ms, err={err}")
-        M.release()
-
-    # tuner choice (cost model on matrix statistics; falls back to the best measured)
-    pick = tuner.pick_block_size(A, candidates=s_list)
-    s_best = pick if pick in per_s else max(per_s, key=lambda k: per_s[k]["gflops"])
-    best = per_s[s_best]
-    plain = per_s.get(64)
-
-    calib = None
-    if not args.no_calib:
-        gbs, ms = calib_copy_gbs(hip)
-        calib = {"copy_gbs": round(gbs, 1), "bytes": 2 * (2 << 30), "ms": round(ms, 3)}
-        log(f"stream copy ceiling {gbs:.0f} GB/s")
-
-    out = {
-        "metric": "SpMV GFLOP/s + % of HBM roofline; learned-tuned vs plain ABHSF, same GPU",
-        "value": best["gflops"],
-        "unit": "GFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": best["ms"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"block_mix n=2^{args.n.bit_length() - 1} (20% 3x dense 32x32, 30% tridiag 8x8, 50% 12/row), "
-                        f"N(0,1) FP64 values, seed 0",
-            "nnz": A.nnz, "s": s_best, "s_measured": s_list, "tuner": tuner.NAME,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": best["gbs"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(best["gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
-        },
-        "plain_s64": None if plain is None else {"gflops": plain["gflops"], "gbs": plain["gbs"]},
-        "tuned_vs_plain": None if plain is None else round(best["gflops"] / plain["gflops"], 3),
-        "per_s": per_s,
-        "copy_calibration": calib,
-    }
-    if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(A, s_best, x)
-    print(json.dumps(out), flush=True)
+    from abhsf_amd import hip
+    run = {"block_mix": run_block_mix, "rmat": run_rmat, "spmm": run_spmm, "batch": run_batch}[args.workload]
+    print(json.dumps(run(args, hip)), flush=True)
 
 
 if __name__ == "__main__":

@@ -151,6 +151,21 @@ abhsf_status abhsf_block_hist(const int64_t *rowptr, const int32_t *col, int64_t
                               int32_t on_device, int32_t kr, int32_t ks, int64_t strip_stride, uint64_t *hist,
                               int64_t hist_cap, uint64_t *nblocks, uint64_t *sampled_nnz, void *stream);
 
+/* ---- Batched launch of many matrices ---------------------------------------
+ * No reference counterpart.  One grid-stride launch processes the work items
+ * of every matrix (each with its own block size), followed by the heavy-strip
+ * fix-up; with use_graph the pair is captured once into a hipGraph and
+ * replayed.  The member handles must stay alive and unchanged while the batch
+ * exists.  FP64 storage only. */
+typedef struct abhsf_batch abhsf_batch;
+abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *stream, abhsf_batch **out);
+/* xs[i] / ys[i]: device vectors of matrix i (host arrays of device pointers). */
+abhsf_status abhsf_batch_set_vectors(abhsf_batch *b, const double *const *xs, double *const *ys);
+/* y_i = alpha * A_i * x_i + beta * y_i for every member, enqueued on the batch stream. */
+abhsf_status abhsf_batch_spmv(abhsf_batch *b, double alpha, double beta, int32_t use_graph);
+abhsf_status abhsf_batch_synchronize(abhsf_batch *b);
+abhsf_status abhsf_batch_destroy(abhsf_batch *b);
+
 /* ---- Tuner cost model: 2 x 64 hidden ReLU MLP, FP32, on the GPU ----------
  * No reference counterpart (the reference searches block sizes exhaustively
  * over storage bits, block_statistics/mmf.h:77-102); these are the learned
