@@ -180,7 +180,7 @@ __device__ inline void coo_csr_r1(const DevMat &A, int sc, uint32_t b0, uint32_t
 // some block of the chunk holds more than ZS entries: the remaining entries
 // are spread one per lane and located by a binary search over the chunk's
 // prefix sums in LDS.  The next chunk's block metadata is loaded ahead.
-template <typename VT, typename BT, typename IT, bool COMP, int ZS>
+template <typename VT, typename BT, typename IT, bool COMP, int ZS, int CH>
 __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_t e, const double *__restrict__ x,
                               const Acc &acc, uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, int lane)
 {
@@ -191,39 +191,61 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
     const IT *__restrict__ idx = static_cast<const IT *>(A.coo_idx);
     const uint16_t *__restrict__ z1 = A.z1[COO];
     const BT *__restrict__ bcol = static_cast<const BT *>(A.bcol[COO]);
-    uint32_t jb = b0 + lane;
-    uint32_t z = jb < b1 ? (uint32_t)z1[jb] + 1u : 0u;
-    uint32_t bc = jb < b1 ? (uint32_t)bcol[jb] : 0u;
+    // CH sub-chunks of 64 blocks per step: all their loads are issued before
+    // any of them is consumed, so several memory round trips overlap.
 #pragma unroll 1
-    for (uint32_t cb = b0; cb < b1; cb += kWave) {
-        const uint32_t jn = cb + kWave + (uint32_t)lane;
-        const bool okn = jn < b1;
-        const uint32_t zn = okn ? (uint32_t)z1[jn] + 1u : 0u;
-        const uint32_t bcn = okn ? (uint32_t)bcol[jn] : 0u;
-        const uint32_t incl = wave_incl_scan_dpp(z);
-        const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
-        const uint32_t st = e + incl - z;
-        const int64_t xb = (int64_t)bc * s;
+    for (uint32_t cb = b0; cb < b1; cb += (uint32_t)(kWave * CH)) {
+        uint32_t z[CH], bc[CH], st[CH];
 #pragma unroll
-        for (int i = 0; i < ZS; i++) {
-            if ((uint32_t)i < z) {
-                const uint32_t id = idx[st + i];
-                const double v = (double)val[st + i];
-                const double xv = x[xb + (id & smask)];
-                double pe;
-                const double p = prod<COMP>(v, xv, pe);
-                acc_atomic<COMP>(acc, id >> k, p, pe);
-            }
+        for (int c = 0; c < CH; c++) {
+            const uint32_t jb = cb + (uint32_t)(c * kWave) + (uint32_t)lane;
+            const bool ok = jb < b1;
+            z[c] = ok ? (uint32_t)z1[jb] + 1u : 0u;
+            bc[c] = ok ? (uint32_t)bcol[jb] : 0u;
         }
-        const uint32_t rem = z > (uint32_t)ZS ? z - (uint32_t)ZS : 0u;
-        if (__builtin_amdgcn_read_exec() & __ballot(rem != 0)) {
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const uint32_t incl = wave_incl_scan_dpp(z[c]);
+            st[c] = e + incl - z[c];
+            e += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        }
+        uint32_t id[CH][ZS];
+        double v[CH][ZS], xv[CH][ZS];
+#pragma unroll
+        for (int c = 0; c < CH; c++)
+#pragma unroll
+            for (int i = 0; i < ZS; i++) {
+                const bool h = (uint32_t)i < z[c];
+                id[c][i] = h ? (uint32_t)idx[st[c] + i] : 0u;
+                v[c][i] = h ? (double)val[st[c] + i] : 0.0;
+            }
+#pragma unroll
+        for (int c = 0; c < CH; c++)
+#pragma unroll
+            for (int i = 0; i < ZS; i++)
+                xv[c][i] = ((uint32_t)i < z[c]) ? x[(int64_t)bc[c] * s + (id[c][i] & smask)] : 0.0;
+#pragma unroll
+        for (int c = 0; c < CH; c++)
+#pragma unroll
+            for (int i = 0; i < ZS; i++)
+                if ((uint32_t)i < z[c]) {
+                    double pe;
+                    const double p = prod<COMP>(v[c][i], xv[c][i], pe);
+                    acc_atomic<COMP>(acc, id[c][i] >> k, p, pe);
+                }
+        // phase B: entries beyond the first ZS of a block, one per lane
+#pragma unroll 1
+        for (int c = 0; c < CH; c++) {
+            const uint32_t rem = z[c] > (uint32_t)ZS ? z[c] - (uint32_t)ZS : 0u;
+            if (__ballot(rem != 0) == 0) continue;
             const uint32_t rincl = wave_incl_scan_dpp(rem);
             const uint32_t RT = (uint32_t)__builtin_amdgcn_readlane((int)rincl, kWave - 1);
             s_end[lane] = rincl;
-            s_bc[lane] = bc;
-            s_st[lane] = st + (uint32_t)ZS;
+            s_bc[lane] = bc[c];
+            s_st[lane] = st[c] + (uint32_t)ZS;
             wave_sync();
-            const int nbk = (int)min<uint32_t>(kWave, b1 - cb);
+            const uint32_t cs = cb + (uint32_t)(c * kWave);
+            const int nbk = (int)min<uint32_t>(kWave, b1 - cs);
 #pragma unroll 1
             for (uint32_t q = 0; q < RT; q += kWave) {
                 const uint32_t t = q + lane;
@@ -234,19 +256,16 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
                         if (s_end[mid] > t) hi = mid; else lo = mid + 1;
                     }
                     const uint32_t pos = s_st[lo] + t - (lo ? s_end[lo - 1] : 0u);
-                    const uint32_t id = idx[pos];
-                    const double v = (double)val[pos];
-                    const double xv = x[(int64_t)s_bc[lo] * s + (id & smask)];
+                    const uint32_t ide = idx[pos];
+                    const double ve = (double)val[pos];
+                    const double xe = x[(int64_t)s_bc[lo] * s + (ide & smask)];
                     double pe;
-                    const double p = prod<COMP>(v, xv, pe);
-                    acc_atomic<COMP>(acc, id >> k, p, pe);
+                    const double p = prod<COMP>(ve, xe, pe);
+                    acc_atomic<COMP>(acc, ide >> k, p, pe);
                 }
             }
             wave_sync();
         }
-        e += T;
-        z = zn;
-        bc = bcn;
     }
 }
 
@@ -365,6 +384,18 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
 // itself when light is null) whose rows are written to y; items n_light + u are
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
+// Workgroups are dispatched to the 8 XCDs round-robin; renumber them so that
+// each XCD (and its L2) receives one contiguous range of work items, which
+// keeps the x windows of neighbouring strips in the same L2.
+__device__ inline int64_t xcd_block_id()
+{
+    constexpr int kXcd = 8;
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t q = nb / kXcd, r = nb % kXcd;
+    const int64_t x = b % kXcd, i = b / kXcd;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // One work item of one matrix (see Balance): the wave accumulates in acc_buf
 // (s doubles, 2s when compensated) and writes y rows or a unit's partials.
 template <typename VT, typename BT, typename IT, typename RT, bool COMP>
@@ -382,8 +413,8 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
     }
     wave_sync();
     if (item < B.n_light) {
-        const int64_t I = B.light ? (int64_t)B.light[item] : item;
-        coo_r1<VT, BT, IT, COMP, 2>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc, s_st,
+        const int64_t I = B.light ? (int64_t)__builtin_amdgcn_readfirstlane((int)B.light[item]) : item;
+        coo_r1<VT, BT, IT, COMP, 2, 4>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc, s_st,
                                     lane);
         coo_csr_r1<VT, BT, IT, RT, COMP>(A, CSR, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc, s_end,
                                          s_bc, lane);
@@ -403,7 +434,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         const uint32_t *U = B.units + u * 4;
         const uint32_t sc = U[0] & 3u, b0 = U[1], b1 = U[2], e0 = U[3];
         if (sc == COO)
-            coo_r1<VT, BT, IT, COMP, 2>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, lane);
+            coo_r1<VT, BT, IT, COMP, 2, 4>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, lane);
         else if (sc == CSR)
             coo_csr_r1<VT, BT, IT, RT, COMP>(A, (int)sc, b0, b1, e0, x, acc, s_end, s_bc, lane);
         else
@@ -431,8 +462,8 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     __shared__ uint32_t s_end[kWaves][kWave];
     __shared__ uint32_t s_bc[kWaves][kWave];
     __shared__ uint32_t s_st[kWaves][kWave];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t item = (int64_t)blockIdx.x * kWaves + wave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t item = xcd_block_id() * kWaves + wave;
     if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
     spmv_item<VT, BT, IT, RT, COMP>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave], s_bc[wave], s_st[wave],
                                     lane);
