@@ -22,7 +22,7 @@ SCHEMES = ("coo", "csr", "bitmap", "dense")
 # every symbol include/abhsf_c.h declares
 EXPORTED = (
     "abhsf_create", "abhsf_build_csr", "abhsf_build_coo", "abhsf_get_stats", "abhsf_spmv",
-    "abhsf_spmm", "abhsf_export", "abhsf_block_hist", "abhsf_mlp_num_params", "abhsf_mlp_forward",
+    "abhsf_spmm", "abhsf_export", "abhsf_import", "abhsf_block_hist", "abhsf_mlp_num_params", "abhsf_mlp_forward",
     "abhsf_mlp_backward", "abhsf_mlp_train", "abhsf_batch_create", "abhsf_batch_set_vectors",
     "abhsf_batch_spmv", "abhsf_batch_synchronize", "abhsf_batch_destroy", "abhsf_synchronize",
     "abhsf_destroy", "abhsf_last_error", "abhsf_version",
@@ -88,6 +88,7 @@ def lib():
         L.abhsf_spmv.argtypes = [vp, d, vp, d, vp]
         L.abhsf_spmm.argtypes = [vp, i32, d, vp, i64, d, vp, i64]
         L.abhsf_export.argtypes = [vp, C.c_char_p, vp, sz, C.POINTER(sz)]
+        L.abhsf_import.argtypes = [vp, i64, i64, i64, i32, vp, vp, vp]
         L.abhsf_block_hist.argtypes = [vp, vp, i64, i64, i64, i32, i32, i32, i64, vp, i64, vp, vp, vp]
         L.abhsf_mlp_num_params.argtypes = [i32]
         L.abhsf_mlp_forward.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp]

@@ -205,6 +205,18 @@ abhsf_status abhsf_build_coo(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
     });
 }
 
+abhsf_status abhsf_import(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz, int32_t count,
+                          const char *const *names, const void *const *data, const size_t *bytes)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h && names && data && bytes && count > 0, ABHSF_ERR_INVALID, "null argument");
+        check_shape(m, n, nnz);
+        reset_storage(h, m, n, nnz);
+        abhsf::import_storage(h->S, count, names, data, bytes, h->stream);
+        finish_build(h);
+    });
+}
+
 abhsf_status abhsf_get_stats(const abhsf_handle *h, abhsf_stats *out)
 {
     return guarded([&] {

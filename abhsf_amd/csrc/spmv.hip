@@ -699,7 +699,6 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
 #pragma unroll 1
             for (int w = 0; w < W; w++) {
                 const uint64_t word = (sc == BITMAP) ? words[w] : ~0ull;
-#pragma unroll
                 for (int t = 0; t < R; t++) {
                     const uint32_t i = (uint32_t)(t * E + slot);  // bit within the word
                     const uint32_t pos = (uint32_t)w * 64u + i;    // position in the block

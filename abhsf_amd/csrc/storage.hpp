@@ -40,4 +40,9 @@ struct Storage {
 void build_from_device_csr(Storage &S, const int64_t *rowptr, const int32_t *col, const double *val,
                            hipStream_t st);
 
+// Recreate S (shape and options already set) from exported arrays in host
+// memory; throws abhsf::Error if the arrays are not a valid storage.
+void import_storage(Storage &S, int32_t count, const char *const *names, const void *const *data,
+                    const size_t *bytes, hipStream_t st);
+
 }  // namespace abhsf

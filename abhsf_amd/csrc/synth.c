@@ -345,3 +345,40 @@ int synth_num_threads(void)
     return 1;
 #endif
 }
+
+/* Uniform block matrix of the reference's micro-benchmarks
+ * (spmv/orig/88.cpp:43-141, 150-202): nb x nb blocks of size bs, every block a
+ * copy of one random pattern of bnnz distinct positions drawn from the seeded
+ * stream; all values 1.  With x = 1 every row sums to the pattern's row
+ * count, so sum(y) = bnnz * nb^2 per SpMV (88.cpp:828-832). */
+int64_t synth_uniform_blocks(int64_t nb, int64_t bs, int64_t bnnz, uint64_t seed, int64_t *rowptr, int32_t *col,
+                             double *val)
+{
+    if (bnnz < 1 || bnnz > bs * bs || bs > 64) return -1;
+    uint8_t pat[64 * 64];
+    memset(pat, 0, sizeof(pat));
+    for (int64_t got = 0, t = 0; got < bnnz; t++) {
+        uint64_t p = rnd(seed, 41, (uint64_t)t) % (uint64_t)(bs * bs);
+        if (!pat[p]) { pat[p] = 1; got++; }
+    }
+    int64_t rowcnt[64];
+    for (int64_t r = 0; r < bs; r++) {
+        rowcnt[r] = 0;
+        for (int64_t c = 0; c < bs; c++) rowcnt[r] += pat[r * bs + c];
+    }
+    const int64_t n = nb * bs;
+    if (rowptr) {
+        for (int64_t r = 0; r < n; r++) rowptr[r] = rowcnt[r % bs] * nb;
+        scan_rowptr(rowptr, n);
+    }
+    if (!col) return rowptr ? rowptr[n] : bnnz * nb * nb;
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < n; r++) {
+        int64_t e = rowptr[r];
+        const int64_t lr = r % bs;
+        for (int64_t J = 0; J < nb; J++)
+            for (int64_t c = 0; c < bs; c++)
+                if (pat[lr * bs + c]) { col[e] = (int32_t)(J * bs + c); val[e++] = 1.0; }
+    }
+    return rowptr[n];
+}

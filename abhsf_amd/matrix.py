@@ -151,6 +151,32 @@ class AbhsfMatrix:
         out["bm_words"] = self.export("bm_words", np.uint64)
         return out
 
+    def save(self, path: str) -> None:
+        """Serialised ABHSF blob: every storage array plus shape and options (npz)."""
+        st = self.stats()
+        arrays = self.export_all()
+        meta = np.array([st["m"], st["n"], st["nnz"], self.opts.block_size, self.opts.policy,
+                         self.opts.precision, self.opts.is_binary], dtype=np.int64)
+        np.savez(path, __meta__=meta, **arrays)
+
+    @classmethod
+    def load(cls, path: str, variant: str = "auto", stream=None) -> "AbhsfMatrix":
+        """Recreate a matrix from save() output without re-running the conversion."""
+        with np.load(path, allow_pickle=False) as f:
+            m, n, nnz, s, pol, prec, binary = (int(v) for v in f["__meta__"])
+            names = [k for k in f.files if k != "__meta__"]
+            arrays = [np.ascontiguousarray(f[k]) for k in names]
+        inv_pol = {v: k for k, v in _POLICY.items()}
+        inv_prec = {v: k for k, v in _PREC.items()}
+        M = cls(block_size=s, policy=inv_pol[pol], precision=inv_prec[prec], variant=variant, stream=stream,
+                is_binary=bool(binary))
+        cn = (C.c_char_p * len(names))(*[nm.encode() for nm in names])
+        cd = (C.c_void_p * len(names))(*[a.ctypes.data for a in arrays])
+        cb = (C.c_size_t * len(names))(*[a.nbytes for a in arrays])
+        capi.call("abhsf_import", M._h, m, n, nnz, len(names), cn, cd, cb)
+        M.props = matrix_properties(m, n, nnz)
+        return M
+
     # -- release ---------------------------------------------------------------
     def release(self) -> None:
         if self._h and self._h.value:

@@ -29,8 +29,9 @@ def _L():
         L.synth_banded.argtypes = [i64, i64, u64, vp, vp, vp]
         L.synth_random_blocks.argtypes = [i64, i64, i64, d, u64, vp, vp, vp]
         L.synth_rmat.argtypes = [C.c_int, i64, d, d, d, u64, vp, vp, vp]
+        L.synth_uniform_blocks.argtypes = [i64, i64, i64, u64, vp, vp, vp]
         for f in ("synth_laplacian2d", "synth_stencil27", "synth_block_mix", "synth_banded",
-                  "synth_random_blocks", "synth_rmat"):
+                  "synth_random_blocks", "synth_rmat", "synth_uniform_blocks"):
             getattr(L, f).restype = i64
         _lib = L
     return _lib
@@ -127,6 +128,14 @@ def rmat(scale: int, ef: int = 16, a: float = 0.57, b: float = 0.19, c: float = 
     n = 1 << scale
     rowptr, col, val = _two_call("synth_rmat", n, scale, ef, a, b, c, seed)
     return Csr(n, n, rowptr, col, val, f"rmat_{scale}_{ef}")
+
+
+def uniform_blocks(nb: int, bs: int, bnnz: int, seed: int = 0x0498BC04) -> Csr:
+    """nb x nb copies of one random bs x bs pattern with bnnz entries, values 1
+    (the micro-benchmark matrices of spmv/orig/88.cpp:43-202)."""
+    n = nb * bs
+    rowptr, col, val = _two_call("synth_uniform_blocks", n, nb, bs, bnnz, seed)
+    return Csr(n, n, rowptr, col, val, f"uniform_blocks_{nb}x{bs}_{bnnz}")
 
 
 def from_triplets(m: int, n: int, rows, cols, vals, name: str = "") -> Csr:

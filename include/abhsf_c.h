@@ -139,6 +139,17 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
 abhsf_status abhsf_export(const abhsf_handle *h, const char *name, void *dst, size_t cap,
                           size_t *bytes);
 
+/* Recreate the storage from arrays obtained with abhsf_export (a serialised
+ * ABHSF blob: count (name, host pointer, bytes) triples covering every array of
+ * the layout above), skipping the conversion.  The handle's options must match
+ * the ones the arrays were built with.  Every array is validated on the host
+ * (sizes, offsets, entries inside the m x n matrix, scheme = cost-model
+ * choice) before upload; an inconsistent blob returns ABHSF_ERR_INVALID.
+ * The reference keeps converted matrices only as text tables or the binary
+ * COO .mm file (utils/mtx2mm/mtx2mm.cpp:101-136). */
+abhsf_status abhsf_import(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz, int32_t count,
+                          const char *const *names, const void *const *data, const size_t *bytes);
+
 /* Block statistics on the GPU: hist[z] = number of nonzero r x s blocks
  * (r = 2^kr, s = 2^ks, 0 <= kr, ks <= 16) holding z entries (z >= hist_cap - 1
  * lands in hist[hist_cap - 1]); *nblocks = number of nonzero blocks.  Only

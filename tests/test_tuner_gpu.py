@@ -113,3 +113,18 @@ def test_trained_model_beats_plain_abhsf_on_heldout(gpu):
         print(f"{A.name}: pick {pk} regret {r_mlp[-1]:.1f}% vs plain s=64 {r_plain[-1]:.1f}%")
     print(f"mean regret: mlp {np.mean(r_mlp):.1f}%, plain s=64 {np.mean(r_plain):.1f}%")
     assert np.mean(r_mlp) < np.mean(r_plain)
+
+
+def test_blockstats_all_64_pairs_and_tables(gpu):
+    """The GPU block-statistics pipeline (abhsf_amd/blockstats.py) reproduces
+    the oracle's 64 histograms; the tables built on them are the ones checked
+    against the reference binaries in tests/test_blockstats_cpu.py."""
+    from abhsf_amd import blockstats
+    for A in (synth.rmat(10, 8, seed=7), synth.block_mix(2048, seed=8)):
+        r, c, _ = A.triplets()
+        hg = blockstats.histograms(A)
+        for (rr, ss), h in hg.items():
+            assert h == pyoracle.block_hist(r, c, rr.bit_length() - 1, ss.bit_length() - 1), (rr, ss)
+        t = blockstats.mmf_table(A, hg)
+        ho = {k: pyoracle.block_hist(r, c, k[0].bit_length() - 1, k[1].bit_length() - 1) for k in hg}
+        assert t == {k: pyoracle.mmf_row(A.m, A.n, A.nnz, k[0], k[1], ho[k]) for k in hg}
