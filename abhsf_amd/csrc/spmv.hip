@@ -384,16 +384,19 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
 // itself when light is null) whose rows are written to y; items n_light + u are
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
-// Workgroups are dispatched to the 8 XCDs round-robin; renumber them so that
-// each XCD (and its L2) receives one contiguous range of work items, which
-// keeps the x windows of neighbouring strips in the same L2.
+// Workgroups are dispatched to the 8 XCDs round-robin.  Renumber them so that
+// each XCD (and its L2) receives runs of kRun consecutive work items (the x
+// windows of neighbouring strips stay in one L2) while the runs rotate over
+// the XCDs (a skewed range, e.g. the units of heavy strips, is still spread
+// over all of them).  A tail shorter than 8 runs keeps the dispatch order.
 __device__ inline int64_t xcd_block_id()
 {
-    constexpr int kXcd = 8;
+    constexpr int64_t kXcd = 8, kRun = 32;
     const int64_t nb = gridDim.x, b = blockIdx.x;
-    const int64_t q = nb / kXcd, r = nb % kXcd;
+    const int64_t full = (nb / (kXcd * kRun)) * (kXcd * kRun);
+    if (b >= full) return b;
     const int64_t x = b % kXcd, i = b / kXcd;
-    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    return ((i / kRun) * kXcd + x) * kRun + (i % kRun);
 }
 
 // One work item of one matrix (see Balance): the wave accumulates in acc_buf
