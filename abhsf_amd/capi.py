@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libabhsf.so")
 OK, ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_UNSUPPORTED = range(6)
 POLICY_PREPROC, POLICY_MSMF = 0, 1
 F64, F32 = 0, 1
-VARIANT_AUTO, VARIANT_STRIP, VARIANT_BALANCED = 0, 1, 2
+VARIANT_AUTO, VARIANT_STRIP, VARIANT_BALANCED, VARIANT_PANEL = 0, 1, 2, 3
 SCHEMES = ("coo", "csr", "bitmap", "dense")
 
 # every symbol include/abhsf_c.h declares
@@ -24,7 +24,7 @@ EXPORTED = (
     "abhsf_create", "abhsf_build_csr", "abhsf_build_coo", "abhsf_get_stats", "abhsf_spmv",
     "abhsf_spmm", "abhsf_export", "abhsf_import", "abhsf_block_hist", "abhsf_mlp_num_params", "abhsf_mlp_forward",
     "abhsf_mlp_backward", "abhsf_mlp_train", "abhsf_batch_create", "abhsf_batch_set_vectors",
-    "abhsf_batch_spmv", "abhsf_batch_synchronize", "abhsf_batch_destroy", "abhsf_synchronize",
+    "abhsf_batch_spmv", "abhsf_batch_synchronize", "abhsf_batch_profile", "abhsf_batch_destroy", "abhsf_synchronize",
     "abhsf_destroy", "abhsf_last_error", "abhsf_version",
 )
 
@@ -98,6 +98,7 @@ def lib():
         L.abhsf_batch_set_vectors.argtypes = [vp, vp, vp]
         L.abhsf_batch_spmv.argtypes = [vp, d, d, i32]
         L.abhsf_batch_synchronize.argtypes = [vp]
+        L.abhsf_batch_profile.argtypes = [vp, d, d, vp]
         L.abhsf_batch_destroy.argtypes = [vp]
         L.abhsf_synchronize.argtypes = [vp]
         L.abhsf_destroy.argtypes = [vp]

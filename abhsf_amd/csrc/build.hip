@@ -17,6 +17,7 @@
 // All arithmetic is integer, so the result is bit-exact and deterministic.
 #include "storage.hpp"
 
+#include <algorithm>
 #include <chrono>
 
 namespace abhsf {
@@ -283,23 +284,29 @@ SpmvLaunch Storage::launch() const
     L.bcol_bytes = bcol_bytes;
     L.val_bytes = val_bytes;
     L.bal.light = bal_light.get<uint32_t>();
+    L.bal.pack = bal_pack.get<uint32_t>();
     L.bal.units = bal_units.get<uint32_t>();
     L.bal.heavy = bal_heavy.get<uint32_t>();
     L.bal.part = bal_part.get<double>();
+    L.bal.order = bal_order.get<uint32_t>();
     L.bal.n_light = n_light;
     L.bal.n_units = n_units;
     L.bal.n_heavy = n_heavy;
+    L.bal.n_slots = n_slots;
     return L;
 }
 
-void Storage::build_balance(int64_t unit, hipStream_t st)
+void Storage::build_balance(int64_t unit, hipStream_t st, int panels)
 {
     bal_light.release();
+    bal_pack.release();
     bal_units.release();
     bal_heavy.release();
     bal_part.release();
-    n_light = n_units = n_heavy = 0;
+    bal_order.release();
+    n_light = n_units = n_heavy = n_slots = 0;
     if (unit <= 0 || M == 0) return;
+    ABHSF_REQUIRE(panels == 1 || panels == kXcdCount, ABHSF_ERR_INVALID, "column panels must match the XCD count");
     std::vector<uint32_t> ep[4], sp[4];
     for (int x = 0; x < 4; x++) {
         ep[x].resize((size_t)M + 1);
@@ -309,27 +316,84 @@ void Storage::build_balance(int64_t unit, hipStream_t st)
     }
     ABHSF_HIP(hipStreamSynchronize(st));
     const int64_t H = 2 * unit;
-    std::vector<uint32_t> light, units, heavy;
-    bool any = false;
-    for (int64_t I = 0; I < M && !any; I++) {
+    const int64_t T = unit / 2;  // light strips are packed into items of up to T entries
+    auto strip_work = [&](int64_t I) {
         int64_t w = 0;
         for (int x = 0; x < 4; x++) w += (int64_t)ep[x][I + 1] - ep[x][I];
-        any = w > H;
+        return w;
+    };
+    if (panels == 1) {  // nothing to split or pack: one item per strip, no tables
+        bool any = false;
+        for (int64_t I = 0; I < M && !any; I++) {
+            const int64_t w = strip_work(I);
+            any = w > H || (I + 1 < M && w + strip_work(I + 1) <= T);
+        }
+        if (!any) return;
     }
-    if (!any) return;
     std::vector<uint16_t> zz[4];
     for (int x = 0; x < 3; x++) {
         zz[x].resize(nb[x]);
         if (nb[x]) ABHSF_HIP(hipMemcpyAsync(zz[x].data(), z1[x].get(), nb[x] * 2, hipMemcpyDeviceToHost, st));
     }
+    // block columns (only needed for panels)
+    std::vector<uint32_t> bc[4];
+    std::vector<uint16_t> bc16;
+    if (panels > 1) {
+        for (int x = 0; x < 4; x++) {
+            bc[x].resize(nb[x]);
+            if (!nb[x]) continue;
+            if (bcol_bytes == 4) {
+                ABHSF_HIP(hipMemcpyAsync(bc[x].data(), bcol[x].get(), nb[x] * 4, hipMemcpyDeviceToHost, st));
+            } else {
+                bc16.resize(nb[x]);
+                ABHSF_HIP(hipMemcpyAsync(bc16.data(), bcol[x].get(), nb[x] * 2, hipMemcpyDeviceToHost, st));
+                ABHSF_HIP(hipStreamSynchronize(st));
+                for (uint64_t b = 0; b < nb[x]; b++) bc[x][b] = bc16[b];
+            }
+        }
+    }
     ABHSF_HIP(hipStreamSynchronize(st));
     const int64_t ss = (int64_t)s * s;
+    auto work = [&](int x, uint32_t b) -> int64_t { return (x == DENSE) ? ss : (int64_t)zz[x][b] + 1; };
+    // panel of every block column: cut where the running work passes p/panels
+    std::vector<uint8_t> panel_of;
+    if (panels > 1) {
+        std::vector<uint64_t> cw((size_t)N, 0);
+        uint64_t total = 0;
+        for (int x = 0; x < 4; x++)
+            for (uint64_t b = 0; b < nb[x]; b++) {
+                cw[bc[x][b]] += (uint64_t)work(x, (uint32_t)b);
+                total += (uint64_t)work(x, (uint32_t)b);
+            }
+        panel_of.resize((size_t)N);
+        uint64_t acc = 0;
+        int p = 0;
+        for (int64_t J = 0; J < N; J++) {
+            panel_of[J] = (uint8_t)p;
+            acc += cw[J];
+            while (p < panels - 1 && acc * (uint64_t)panels >= total * (uint64_t)(p + 1)) p++;
+        }
+    }
+    std::vector<uint32_t> light, units, heavy;
+    std::vector<uint8_t> light_panel, unit_panel;
+    std::vector<int64_t> light_work;
     light.reserve((size_t)M);
     for (int64_t I = 0; I < M; I++) {
         int64_t w = 0;
-        for (int x = 0; x < 4; x++) w += (int64_t)ep[x][I + 1] - ep[x][I];
-        if (w <= H) {
+        int p0 = -1;
+        bool multi = false;
+        for (int x = 0; x < 4; x++) {
+            w += (int64_t)ep[x][I + 1] - ep[x][I];
+            if (panels > 1)
+                for (uint32_t b = sp[x][I]; b < sp[x][I + 1] && !multi; b++) {
+                    const int p = panel_of[bc[x][b]];
+                    if (p0 < 0) p0 = p; else multi = p != p0;
+                }
+        }
+        if (w <= H && !multi) {
             light.push_back((uint32_t)I);
+            light_panel.push_back((uint8_t)(p0 >= 0 ? p0 : (int)(I % panels)));
+            light_work.push_back(w);
             continue;
         }
         const uint32_t first = (uint32_t)(units.size() / 4);
@@ -337,33 +401,97 @@ void Storage::build_balance(int64_t unit, hipStream_t st)
             const uint32_t b0 = sp[x][I], b1 = sp[x][I + 1];
             uint64_t e = ep[x][I], ue = e;
             uint32_t ub = b0;
+            int up = 0;
             int64_t acc = 0;
             for (uint32_t b = b0; b < b1; b++) {
-                const int64_t wz = (x == DENSE) ? ss : (int64_t)zz[x][b] + 1;
-                if (acc > 0 && acc + wz > unit) {
+                const int64_t wz = work(x, b);
+                const int p = panels > 1 ? panel_of[bc[x][b]] : 0;
+                if (b > ub && (p != up || acc + wz > unit)) {
                     units.insert(units.end(), {(uint32_t)x, ub, b, (uint32_t)ue});
+                    unit_panel.push_back((uint8_t)up);
                     ub = b;
                     ue = e;
                     acc = 0;
                 }
+                if (b == ub) up = p;
                 acc += wz;
                 e += (uint64_t)wz;
             }
-            if (ub < b1) units.insert(units.end(), {(uint32_t)x, ub, b1, (uint32_t)ue});
+            if (ub < b1) {
+                units.insert(units.end(), {(uint32_t)x, ub, b1, (uint32_t)ue});
+                unit_panel.push_back((uint8_t)up);
+            }
         }
         heavy.insert(heavy.end(), {(uint32_t)I, first, (uint32_t)(units.size() / 4 - first)});
     }
-    n_light = (int64_t)light.size();
+    // pack consecutive light strips (of one panel) into items of <= T entries
+    // and at most kPackMax strips
+    constexpr size_t kPackMax = 64;
+    if (panels > 1) {  // group the light strips by panel, strip order kept inside a panel
+        std::vector<size_t> idx(light.size());
+        for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return light_panel[a] < light_panel[b]; });
+        std::vector<uint32_t> l2(light.size());
+        std::vector<uint8_t> p2(light.size());
+        std::vector<int64_t> w2(light.size());
+        for (size_t i = 0; i < idx.size(); i++) {
+            l2[i] = light[idx[i]];
+            p2[i] = light_panel[idx[i]];
+            w2[i] = light_work[idx[i]];
+        }
+        light.swap(l2);
+        light_panel.swap(p2);
+        light_work.swap(w2);
+    }
+    std::vector<uint32_t> pack;
+    std::vector<uint8_t> item_panel;
+    {
+        int64_t iw = 0;
+        for (size_t i = 0; i < light.size(); i++) {
+            const bool open = !pack.empty() && i > pack.back();
+            const bool fits = open && light_panel[i] == item_panel.back() && iw + light_work[i] <= T &&
+                              i - pack.back() < kPackMax;
+            if (!fits) {
+                pack.push_back((uint32_t)i);
+                item_panel.push_back(light_panel[i]);
+                iw = 0;
+            }
+            iw += light_work[i];
+        }
+        pack.push_back((uint32_t)light.size());
+    }
+    n_light = (int64_t)pack.size() - 1;
     n_units = (int64_t)(units.size() / 4);
     n_heavy = (int64_t)(heavy.size() / 3);
+    ABHSF_REQUIRE(n_light + n_units < (int64_t)kNoItem, ABHSF_ERR_UNSUPPORTED, "too many work items");
+    std::vector<uint32_t> order;
+    if (panels > 1) {
+        std::vector<std::vector<uint32_t>> lists((size_t)panels);
+        for (int64_t i = 0; i < n_light; i++) lists[item_panel[i]].push_back((uint32_t)i);
+        for (int64_t u = 0; u < n_units; u++) lists[unit_panel[u]].push_back((uint32_t)(n_light + u));
+        size_t G = 0;
+        for (auto &l : lists) G = std::max(G, (l.size() + 3) / 4);
+        order.assign(G * (size_t)panels * 4, kNoItem);
+        for (int p = 0; p < panels; p++)
+            for (size_t j = 0; j < lists[p].size(); j++)
+                order[((j / 4) * (size_t)panels + (size_t)p) * 4 + j % 4] = lists[p][j];
+        n_slots = (int64_t)order.size();
+    }
     bal_light.alloc(light.size() * 4);
+    bal_pack.alloc(pack.size() * 4);
+    ABHSF_HIP(hipMemcpyAsync(bal_pack.get(), pack.data(), pack.size() * 4, hipMemcpyHostToDevice, st));
     bal_units.alloc(units.size() * 4);
     bal_heavy.alloc(heavy.size() * 4);
+    bal_order.alloc(order.size() * 4);
     bal_part.alloc((size_t)n_units * (size_t)s * sizeof(double) * (val_bytes == 4 ? 2 : 1));
     if (!light.empty())
         ABHSF_HIP(hipMemcpyAsync(bal_light.get(), light.data(), light.size() * 4, hipMemcpyHostToDevice, st));
-    ABHSF_HIP(hipMemcpyAsync(bal_units.get(), units.data(), units.size() * 4, hipMemcpyHostToDevice, st));
-    ABHSF_HIP(hipMemcpyAsync(bal_heavy.get(), heavy.data(), heavy.size() * 4, hipMemcpyHostToDevice, st));
+    if (!units.empty())
+        ABHSF_HIP(hipMemcpyAsync(bal_units.get(), units.data(), units.size() * 4, hipMemcpyHostToDevice, st));
+    if (!heavy.empty())
+        ABHSF_HIP(hipMemcpyAsync(bal_heavy.get(), heavy.data(), heavy.size() * 4, hipMemcpyHostToDevice, st));
+    if (!order.empty())
+        ABHSF_HIP(hipMemcpyAsync(bal_order.get(), order.data(), order.size() * 4, hipMemcpyHostToDevice, st));
     ABHSF_HIP(hipStreamSynchronize(st));
 }
 
@@ -377,7 +505,8 @@ uint64_t Storage::value_bytes() const
 uint64_t Storage::stored_bytes() const
 {
     uint64_t b = value_bytes() + coo_idx.bytes() + csr_rowptr.bytes() + csr_col.bytes() + bm_words.bytes() +
-                 bal_light.bytes() + bal_units.bytes() + bal_heavy.bytes();
+                 bal_light.bytes() + bal_pack.bytes() + bal_units.bytes() + bal_heavy.bytes() +
+                 bal_order.bytes();
     for (int x = 0; x < 4; x++) b += sptr[x].bytes() + eptr[x].bytes() + bcol[x].bytes() + z1[x].bytes();
     return b;
 }

@@ -34,7 +34,7 @@ struct abhsf_batch {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<abhsf::BatchEntry> host;
-    abhsf::DevBuf table, heavy;
+    abhsf::DevBuf table, heavy, item_mat;
     int64_t total_items = 0, nheavy = 0;
     int grid = 1;
     bool vectors_set = false;
@@ -92,7 +92,9 @@ int64_t balance_unit(const abhsf::Storage &S) { return std::max<int64_t>(2048, 8
 
 void finish_build(abhsf_handle *h)
 {
-    if (h->opts.variant != ABHSF_VARIANT_STRIP) h->S.build_balance(balance_unit(h->S), h->stream);
+    if (h->opts.variant != ABHSF_VARIANT_STRIP)
+        h->S.build_balance(balance_unit(h->S), h->stream,
+                           h->opts.variant == ABHSF_VARIANT_PANEL ? abhsf::kXcdCount : 1);
 }
 
 void check_shape(int64_t m, int64_t n, int64_t nnz)
@@ -120,7 +122,7 @@ abhsf_status abhsf_create(const abhsf_options *opts, abhsf_handle **out)
                       "unknown policy");
         ABHSF_REQUIRE(opts->precision == ABHSF_F64 || opts->precision == ABHSF_F32, ABHSF_ERR_INVALID,
                       "unknown precision");
-        ABHSF_REQUIRE(opts->variant >= ABHSF_VARIANT_AUTO && opts->variant <= ABHSF_VARIANT_BALANCED, ABHSF_ERR_INVALID,
+        ABHSF_REQUIRE(opts->variant >= ABHSF_VARIANT_AUTO && opts->variant <= ABHSF_VARIANT_PANEL, ABHSF_ERR_INVALID,
                       "unknown variant");
         *out = nullptr;
         abhsf_handle *h = new abhsf_handle();
@@ -360,6 +362,16 @@ abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *
         }
         b->total_items = item;
         b->nheavy = (int64_t)hv.size() / 2;
+        ABHSF_REQUIRE(item < (int64_t)UINT32_MAX, ABHSF_ERR_UNSUPPORTED, "too many work items in one batch");
+        std::vector<uint32_t> im((size_t)item);
+        for (size_t i = 0; i < b->host.size(); i++) {
+            const int64_t i0 = b->host[i].item0;
+            const int64_t i1 = (i + 1 < b->host.size()) ? b->host[i + 1].item0 : item;
+            std::fill(im.begin() + i0, im.begin() + i1, (uint32_t)i);
+        }
+        b->item_mat.alloc(im.size() * 4);
+        if (!im.empty())
+            ABHSF_HIP(hipMemcpyAsync(b->item_mat.get(), im.data(), im.size() * 4, hipMemcpyHostToDevice, b->stream));
         const int64_t want = (item + 3) / 4;
         b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 8192));
         b->table.alloc(b->host.size() * sizeof(abhsf::BatchEntry));
@@ -392,7 +404,7 @@ abhsf_status abhsf_batch_spmv(abhsf_batch *b, double alpha, double beta, int32_t
         ABHSF_REQUIRE(b, ABHSF_ERR_INVALID, "null batch");
         ABHSF_REQUIRE(b->vectors_set, ABHSF_ERR_STATE, "abhsf_batch_set_vectors not called");
         auto launch = [&] {
-            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), (int)b->host.size(), b->total_items,
+            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
                               b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->grid, b->stream);
         };
         if (!use_graph) {
@@ -426,6 +438,22 @@ abhsf_status abhsf_batch_synchronize(abhsf_batch *b)
 {
     return guarded([&] {
         ABHSF_REQUIRE(b, ABHSF_ERR_INVALID, "null batch");
+        ABHSF_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+abhsf_status abhsf_batch_profile(abhsf_batch *b, double alpha, double beta, uint64_t *cycles)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(b && cycles, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(b->vectors_set, ABHSF_ERR_STATE, "abhsf_batch_set_vectors not called");
+        const size_t nm = b->host.size();
+        abhsf::DevBuf cyc(nm * sizeof(unsigned long long));
+        ABHSF_HIP(hipMemsetAsync(cyc.get(), 0, cyc.bytes(), b->stream));
+        abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
+                          b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->grid, b->stream,
+                          cyc.get<unsigned long long>());
+        ABHSF_HIP(hipMemcpyAsync(cycles, cyc.get(), cyc.bytes(), hipMemcpyDeviceToHost, b->stream));
         ABHSF_HIP(hipStreamSynchronize(b->stream));
     });
 }

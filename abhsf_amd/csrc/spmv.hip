@@ -115,61 +115,61 @@ __device__ inline void group_reduce(int P, double &hi, double &lo)
     }
 }
 
-// COO or CSR group of strip I, element-parallel (one entry per lane per step).
-// Blocks [b0, b1) of scheme sc, whose first value is at offset e.
-template <typename VT, typename BT, typename IT, typename RT, bool COMP>
-__device__ inline void coo_csr_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1, uint32_t e,
-                                  const double *__restrict__ x, const Acc &acc, uint32_t *s_end, uint32_t *s_bc,
-                                  int lane)
+// CSR group, blocks [b0, b1) whose first value is at offset e.  Lanes own
+// block rows: for s <= 64 a wave step covers 64/s blocks (lane l -> block l/s,
+// row l%s), for s > 64 one block in s/64 row passes.  A lane reads its row's
+// bounds from the block's row pointers, sums the row's products in registers
+// and adds the sum to the strip accumulator: no search, one LDS update per
+// row.  s_st holds the chunk's exclusive value offsets, s_bc its block columns.
+template <typename VT, typename BT, typename RT, bool COMP>
+__device__ inline void csr_rows_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_t e,
+                                   const double *__restrict__ x, const Acc &acc, uint32_t *s_st, uint32_t *s_bc,
+                                   int lane)
 {
     if (b0 >= b1) return;
-    const int s = A.s, k = A.k;
-    const uint32_t smask = (uint32_t)s - 1;
-    const VT *__restrict__ val = static_cast<const VT *>(A.val[sc]);
+    const int s = A.s;
+    const int sw = s < kWave ? s : kWave;          // lanes per block
+    const int G = kWave / sw;                      // blocks per wave step
+    const int passes = s / sw;                     // row passes per block
+    const VT *__restrict__ val = static_cast<const VT *>(A.val[CSR]);
+    const uint8_t *__restrict__ ccol = A.csr_col;
+    const RT *__restrict__ rowptr = static_cast<const RT *>(A.csr_rowptr);
 #pragma unroll 1
     for (uint32_t cb = b0; cb < b1; cb += kWave) {
         const uint32_t jb = cb + lane;
         const bool ok = jb < b1;
-        const uint32_t z = ok ? (uint32_t)A.z1[sc][jb] + 1u : 0u;
+        const uint32_t z = ok ? (uint32_t)A.z1[CSR][jb] + 1u : 0u;
         const uint32_t incl = wave_incl_scan_dpp(z);
-        const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
-        s_end[lane] = incl;
-        s_bc[lane] = ok ? (uint32_t)ld<BT>(A.bcol[sc], jb) : 0u;
+        s_st[lane] = e + incl - z;
+        s_bc[lane] = ok ? (uint32_t)ld<BT>(A.bcol[CSR], jb) : 0u;
+        e += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
         wave_sync();
         const int nbk = (int)min<uint32_t>(kWave, b1 - cb);
+        const int lb = lane / sw, lr = lane % sw;
 #pragma unroll 1
-        for (uint32_t q = 0; q < T; q += kWave) {
-            const uint32_t t = q + lane;
-            if (t < T) {
-                int lo = 0, hi = nbk - 1;  // first chunk block with end > t
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_end[mid] > t) hi = mid; else lo = mid + 1;
-                }
-                uint32_t lr, lc;
-                if (sc == COO) {
-                    const uint32_t id = ld<IT>(A.coo_idx, e + t);
-                    lr = id >> k;
-                    lc = id & smask;
-                } else {
-                    const uint32_t qb = t - (lo ? s_end[lo - 1] : 0u);
-                    const RT *rp = static_cast<const RT *>(A.csr_rowptr) + (uint64_t)(cb + lo) * (s + 1);
-                    int rl = 0, rh = s - 1;
-                    while (rl < rh) {
-                        const int mid = (rl + rh + 1) >> 1;
-                        if ((uint32_t)rp[mid] <= qb) rl = mid; else rh = mid - 1;
+        for (int j0 = 0; j0 < nbk; j0 += G) {
+            const int j = j0 + lb;
+            if (j < nbk) {
+                const RT *rp = rowptr + (uint64_t)(cb + (uint32_t)j) * (uint64_t)(s + 1);
+                const uint32_t st = s_st[j];
+                const int64_t xc = (int64_t)s_bc[j] * s;
+#pragma unroll 1
+                for (int ps = 0; ps < passes; ps++) {
+                    const int r = ps * sw + lr;
+                    const uint32_t q0 = rp[r], q1 = rp[r + 1];
+                    double hi = 0.0, lo = 0.0;
+                    uint32_t q = q0;
+                    for (; q + 1 < q1; q += 2) {
+                        const double v0 = (double)val[st + q], v1 = (double)val[st + q + 1];
+                        const double x0 = x[xc + ccol[st + q]], x1 = x[xc + ccol[st + q + 1]];
+                        acc_add<COMP>(hi, lo, v0, x0);
+                        acc_add<COMP>(hi, lo, v1, x1);
                     }
-                    lr = (uint32_t)rl;
-                    lc = A.csr_col[e + t];
+                    if (q < q1) acc_add<COMP>(hi, lo, (double)val[st + q], x[xc + ccol[st + q]]);
+                    if (q1 > q0) acc_atomic<COMP>(acc, (uint32_t)r, hi, lo);
                 }
-                const double v = (double)val[e + t];
-                const double xv = x[(int64_t)s_bc[lo] * s + lc];
-                double pe;
-                const double p = prod<COMP>(v, xv, pe);
-                acc_atomic<COMP>(acc, lr, p, pe);
             }
         }
-        e += T;
         wave_sync();
     }
 }
@@ -246,23 +246,41 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
             wave_sync();
             const uint32_t cs = cb + (uint32_t)(c * kWave);
             const int nbk = (int)min<uint32_t>(kWave, b1 - cs);
+            // two windows of 64 entries per step: both searches, then all
+            // loads, then the updates
 #pragma unroll 1
-            for (uint32_t q = 0; q < RT; q += kWave) {
-                const uint32_t t = q + lane;
-                if (t < RT) {
+            for (uint32_t q = 0; q < RT; q += 2 * kWave) {
+                uint32_t pos[2], xb[2], ide[2];
+                double ve[2], xe[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const uint32_t t = q + (uint32_t)(u * kWave + lane);
                     int lo = 0, hi = nbk - 1;
                     while (lo < hi) {
                         const int mid = (lo + hi) >> 1;
                         if (s_end[mid] > t) hi = mid; else lo = mid + 1;
                     }
-                    const uint32_t pos = s_st[lo] + t - (lo ? s_end[lo - 1] : 0u);
-                    const uint32_t ide = idx[pos];
-                    const double ve = (double)val[pos];
-                    const double xe = x[(int64_t)s_bc[lo] * s + (ide & smask)];
-                    double pe;
-                    const double p = prod<COMP>(ve, xe, pe);
-                    acc_atomic<COMP>(acc, ide >> k, p, pe);
+                    pos[u] = s_st[lo] + t - (lo ? s_end[lo - 1] : 0u);
+                    xb[u] = s_bc[lo];
                 }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const bool h = q + (uint32_t)(u * kWave + lane) < RT;
+                    ide[u] = h ? (uint32_t)idx[pos[u]] : 0u;
+                    ve[u] = h ? (double)val[pos[u]] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const bool h = q + (uint32_t)(u * kWave + lane) < RT;
+                    xe[u] = h ? x[(int64_t)xb[u] * s + (ide[u] & smask)] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+                    if (q + (uint32_t)(u * kWave + lane) < RT) {
+                        double pe;
+                        const double p = prod<COMP>(ve[u], xe[u], pe);
+                        acc_atomic<COMP>(acc, ide[u] >> k, p, pe);
+                    }
             }
             wave_sync();
         }
@@ -380,8 +398,9 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
     }
 }
 
-// Work items: items [0, n_light) are whole strips (light[item], or the item
-// itself when light is null) whose rows are written to y; items n_light + u are
+// Work items: items [0, n_light) are light items -- whole strips light[p] for
+// p in [pack[item], pack[item + 1]) (p = item without pack; strip p without
+// light) -- whose rows are written to y; items n_light + u are
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
 // Workgroups are dispatched to the 8 XCDs round-robin.  Renumber them so that
@@ -416,21 +435,39 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
     }
     wave_sync();
     if (item < B.n_light) {
-        const int64_t I = B.light ? (int64_t)__builtin_amdgcn_readfirstlane((int)B.light[item]) : item;
-        coo_r1<VT, BT, IT, COMP, 2, 4>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc, s_st,
-                                    lane);
-        coo_csr_r1<VT, BT, IT, RT, COMP>(A, CSR, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc, s_end,
-                                         s_bc, lane);
-        wave_sync();
-        bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x, acc, lane);
-        bmdn_r1<VT, BT, COMP>(A, DENSE, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], 0, x, acc, lane);
-        wave_sync();
-        for (int r = lane; r < s; r += kWave) {
-            const int64_t gr = I * s + r;
-            if (gr >= A.m) break;
-            double a = acc.hi[r];
-            if constexpr (COMP) a += acc.lo[r];
-            y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * y[gr];
+        // light item: light strips at positions [p0, p1) of the light list
+        uint32_t p0 = (uint32_t)item, p1 = (uint32_t)item + 1u;
+        if (B.pack) {
+            p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item]);
+            p1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item + 1]);
+        }
+#pragma unroll 1
+        for (uint32_t p = p0; p < p1; p++) {
+            const int64_t I = B.light ? (int64_t)__builtin_amdgcn_readfirstlane((int)B.light[p]) : (int64_t)p;
+            if (p != p0) {  // the previous strip's rows are written; clear for this one
+                for (int i = lane; i < s; i += kWave) {
+                    acc.hi[i] = 0.0;
+                    if constexpr (COMP) acc.lo[i] = 0.0;
+                }
+                wave_sync();
+            }
+            coo_r1<VT, BT, IT, COMP, 2, 4>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc,
+                                        s_st, lane);
+            csr_rows_r1<VT, BT, RT, COMP>(A, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc, s_st, s_bc,
+                                          lane);
+            wave_sync();
+            bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x, acc,
+                                  lane);
+            bmdn_r1<VT, BT, COMP>(A, DENSE, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], 0, x, acc, lane);
+            wave_sync();
+            for (int r = lane; r < s; r += kWave) {
+                const int64_t gr = I * s + r;
+                if (gr >= A.m) break;
+                double a = acc.hi[r];
+                if constexpr (COMP) a += acc.lo[r];
+                y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * y[gr];
+            }
+            wave_sync();
         }
     } else {
         const int64_t u = item - B.n_light;
@@ -439,7 +476,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         if (sc == COO)
             coo_r1<VT, BT, IT, COMP, 2, 4>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, lane);
         else if (sc == CSR)
-            coo_csr_r1<VT, BT, IT, RT, COMP>(A, (int)sc, b0, b1, e0, x, acc, s_end, s_bc, lane);
+            csr_rows_r1<VT, BT, RT, COMP>(A, b0, b1, e0, x, acc, s_st, s_bc, lane);
         else
             bmdn_r1<VT, BT, COMP>(A, (int)sc, b0, b1, e0, x, acc, lane);
         wave_sync();
@@ -452,8 +489,9 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
     wave_sync();  // the next item reuses the LDS buffers
 }
 
-// Work items: items [0, n_light) are whole strips (light[item], or the item
-// itself when light is null) whose rows are written to y; items n_light + u are
+// Work items: items [0, n_light) are light items -- whole strips light[p] for
+// p in [pack[item], pack[item + 1]) (p = item without pack; strip p without
+// light) -- whose rows are written to y; items n_light + u are
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
 template <typename VT, typename BT, typename IT, typename RT, bool COMP>
@@ -466,18 +504,29 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     __shared__ uint32_t s_bc[kWaves][kWave];
     __shared__ uint32_t s_st[kWaves][kWave];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t item = xcd_block_id() * kWaves + wave;
+    int64_t item;
+    if (B.order) {  // column panels: the slot table already places items on XCDs
+        const uint32_t o = B.order[(int64_t)blockIdx.x * kWaves + wave];
+        if (o == kNoItem) return;
+        item = (int64_t)__builtin_amdgcn_readfirstlane((int)o);
+    } else {
+        item = xcd_block_id() * kWaves + wave;
+    }
     if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
     spmv_item<VT, BT, IT, RT, COMP>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave], s_bc[wave], s_st[wave],
                                     lane);
 }
 
 // Many matrices in one launch: wave w takes global items w, w + W, ... (W =
-// all waves of the grid); the matrix of an item is found by binary search over
-// the entries' first-item offsets; the block-index types are dispatched per
-// matrix.  FP64 values.
-__global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict__ E, int nmat, int64_t total,
-                                                    double alpha, double beta)
+// all waves of the grid); item_mat[g] names the matrix of global item g (one
+// load instead of a search over the table); the block-index types are
+// dispatched per matrix.  FP64 values.  TIMED: each wave also adds the
+// shader-clock cycles of its item to cyc[matrix] (the per-matrix cost inside
+// a full launch).
+template <bool TIMED>
+__global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict__ E,
+                                                    const uint32_t *__restrict__ item_mat, int64_t total,
+                                                    double alpha, double beta, unsigned long long *cyc)
 {
     constexpr int kWaves = 4;
     __shared__ double s_acc[kWaves][256];
@@ -488,16 +537,13 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
     const int64_t nw = (int64_t)gridDim.x * kWaves;
 #pragma unroll 1
     for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < total; g += nw) {
-        int lo = 0, hi = nmat - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (E[mid].item0 <= g) lo = mid; else hi = mid - 1;
-        }
-        lo = __builtin_amdgcn_readfirstlane(lo);
+        const int lo = __builtin_amdgcn_readfirstlane((int)item_mat[g]);
         const BatchEntry &e = E[lo];
         const int64_t it = g - e.item0;
         double *acc = s_acc[wave];
         uint32_t *se = s_end[wave], *sb = s_bc[wave], *ss = s_st[wave];
+        long long t0 = 0;
+        if constexpr (TIMED) t0 = clock64();
         switch (e.code) {
         case 0: spmv_item<double, uint16_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
         case 1: spmv_item<double, uint16_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
@@ -506,51 +552,103 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
         case 4: spmv_item<double, uint32_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
         default: spmv_item<double, uint32_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
         }
+        if constexpr (TIMED) {
+            const long long t1 = clock64();
+            if (lane == 0) atomicAdd(&cyc[lo], (unsigned long long)(t1 - t0));
+        }
+    }
+}
+
+// Rows of one heavy strip I: y = alpha * (sum of the partial rows of units
+// [u0, u0 + nu)) + beta * y, by a workgroup of kFixThreads threads.  The
+// L = kFixThreads / s threads of row r take units l, l + L, l + 2L, ... in
+// four interleaved chains (several loads in flight); the chains and then the
+// L threads are combined in a fixed order, so the sum does not depend on
+// timing.  Partials of unit u: part[u * s * (COMP ? 2 : 1) + r] (+ s: the
+// compensation terms).
+constexpr int kFixThreads = 1024;
+
+template <bool COMP>
+__device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
+{
+    if constexpr (COMP) {
+        double s2, e2;
+        two_sum(hi, v, s2, e2);
+        hi = s2;
+        lo += vlo + e2;
+    } else {
+        hi += v;
+    }
+}
+
+template <bool COMP>
+__device__ inline void fixup_strip(int s, int64_t m, int64_t I, const double *__restrict__ part, uint32_t u0,
+                                   uint32_t nu, double *__restrict__ y, double alpha, double beta, double *sh)
+{
+    const uint32_t L = (uint32_t)(kFixThreads / s);
+    const int r = (int)(threadIdx.x / L);
+    const uint32_t l = threadIdx.x % L;
+    const size_t stride = (size_t)s * (COMP ? 2 : 1);
+    double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+    uint32_t u = l;
+    for (; u + 3u * L < nu; u += 4u * L) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double *p = part + (size_t)(u0 + u + (uint32_t)j * L) * stride;
+            fix_add<COMP>(hi[j], lo[j], p[r], COMP ? p[s + r] : 0.0);
+        }
+    }
+    for (; u < nu; u += L) {
+        const double *p = part + (size_t)(u0 + u) * stride;
+        fix_add<COMP>(hi[0], lo[0], p[r], COMP ? p[s + r] : 0.0);
+    }
+#pragma unroll
+    for (int j = 1; j < 4; j++) fix_add<COMP>(hi[0], lo[0], hi[j], lo[j]);
+    sh[threadIdx.x] = hi[0];
+    if constexpr (COMP) sh[kFixThreads + threadIdx.x] = lo[0];
+    __syncthreads();
+    if (l == 0) {
+        double H = 0.0, Lo = 0.0;
+        for (uint32_t t = 0; t < L; t++)
+            fix_add<COMP>(H, Lo, sh[(uint32_t)r * L + t], COMP ? sh[kFixThreads + (uint32_t)r * L + t] : 0.0);
+        const int64_t gr = I * s + r;
+        if (gr < m) {
+            const double a = H + Lo;
+            y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * y[gr];
+        }
     }
 }
 
 // Heavy strips of every matrix: hv[2h] = matrix, hv[2h+1] = heavy index.
-__global__ __launch_bounds__(64) void k_batch_fixup(const BatchEntry *__restrict__ E, const uint32_t *__restrict__ hv,
-                                                    double alpha, double beta)
+// TIMED: thread 0 adds the workgroup's cycles times its wave count.
+template <bool TIMED>
+__global__ __launch_bounds__(kFixThreads) void k_batch_fixup(const BatchEntry *__restrict__ E,
+                                                             const uint32_t *__restrict__ hv, double alpha,
+                                                             double beta, unsigned long long *cyc)
 {
-    const BatchEntry &e = E[hv[2 * blockIdx.x]];
+    __shared__ double sh[kFixThreads];
+    long long t0 = 0;
+    if constexpr (TIMED) t0 = clock64();
+    const uint32_t mat = hv[2 * blockIdx.x];
+    const BatchEntry &e = E[mat];
     const uint32_t h = hv[2 * blockIdx.x + 1];
-    const int s = e.A.s;
-    const uint32_t I = e.B.heavy[h * 3], u0 = e.B.heavy[h * 3 + 1], nu = e.B.heavy[h * 3 + 2];
-    for (int r = threadIdx.x; r < s; r += blockDim.x) {
-        const int64_t gr = (int64_t)I * s + r;
-        if (gr >= e.A.m) break;
-        double a = 0.0;
-        for (uint32_t u = u0; u < u0 + nu; u++) a += e.B.part[(size_t)u * s + r];
-        e.y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * e.y[gr];
+    fixup_strip<false>(e.A.s, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1], e.B.heavy[h * 3 + 2], e.y,
+                       alpha, beta, sh);
+    if constexpr (TIMED) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            atomicAdd(&cyc[mat], (unsigned long long)(clock64() - t0) * (unsigned long long)(kFixThreads / kWave));
     }
 }
 
-// Rows of heavy strips: sum the units' partials in unit order (deterministic).
+// Rows of heavy strips (one workgroup per heavy strip).
 template <bool COMP>
-__global__ __launch_bounds__(256) void k_spmv_fixup(int s, int64_t m, double *__restrict__ y, double alpha,
-                                                    double beta, Balance B)
+__global__ __launch_bounds__(kFixThreads) void k_spmv_fixup(int s, int64_t m, double *__restrict__ y, double alpha,
+                                                            double beta, Balance B)
 {
+    __shared__ double sh[kFixThreads * (COMP ? 2 : 1)];
     const int64_t h = blockIdx.x;
-    const uint32_t I = B.heavy[h * 3], u0 = B.heavy[h * 3 + 1], nu = B.heavy[h * 3 + 2];
-    for (int r = threadIdx.x; r < s; r += blockDim.x) {
-        const int64_t gr = (int64_t)I * s + r;
-        if (gr >= m) break;
-        double hi = 0.0, lo = 0.0;
-        for (uint32_t u = u0; u < u0 + nu; u++) {
-            const double *part = B.part + (size_t)u * s * (COMP ? 2 : 1);
-            if constexpr (COMP) {
-                double s2, e2;
-                two_sum(hi, part[r], s2, e2);
-                hi = s2;
-                lo += part[s + r] + e2;
-            } else {
-                hi += part[r];
-            }
-        }
-        const double a = hi + lo;
-        y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * y[gr];
-    }
+    fixup_strip<COMP>(s, m, B.heavy[h * 3], B.part, B.heavy[h * 3 + 1], B.heavy[h * 3 + 2], y, alpha, beta, sh);
 }
 
 // ===========================================================================
@@ -558,8 +656,8 @@ __global__ __launch_bounds__(256) void k_spmv_fixup(int s, int64_t m, double *__
 // ===========================================================================
 // Dense s x s block times an s x R slice of X on the FP64 matrix cores
 // (v_mfma_f64_16x16x4f64: A 16x4 with lane l holding A[l%16][l/16], B 4x16
-// with lane l holding B[l/16][l%16], D 16x16 with lane l holding
-// D[4*(l/16)+r][l%16], r = 0..3).  The 16 x 16 product tiles are added to the
+// with lane l holding B[l/16][l%16], D 16x16 with register r of lane l
+// holding D[4r + l/16][l%16], r = 0..3; layout measured on gfx950).  The 16 x 16 product tiles are added to the
 // strip accumulator (one lane per element, so no atomics).
 template <typename VT, int R>
 __device__ inline void dense_block_mfma(const VT *__restrict__ D, int s, int64_t c0, int64_t n,
@@ -580,11 +678,11 @@ __device__ inline void dense_block_mfma(const VT *__restrict__ D, int s, int64_t
                 const double bx = (gc < n && j < nrhs) ? X[gc * ldx + j] : 0.0;
                 c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bx, c, 0, 0, 0);
             }
-            const int rb = r0 + 4 * lk;
-            acc[(rb + 0) * R + j] += c[0];
-            acc[(rb + 1) * R + j] += c[1];
-            acc[(rb + 2) * R + j] += c[2];
-            acc[(rb + 3) * R + j] += c[3];
+            // v_mfma_f64_16x16x4f64 result: register r of lane l holds row 4r + l/16, column l%16
+            acc[(r0 + lk) * R + j] += c[0];
+            acc[(r0 + 4 + lk) * R + j] += c[1];
+            acc[(r0 + 8 + lk) * R + j] += c[2];
+            acc[(r0 + 12 + lk) * R + j] += c[3];
         }
     }
 }
@@ -783,13 +881,13 @@ void launch_spmv(const DevMat &A, const Balance &B, const double *x, double *y, 
     Balance b = B;
     if (!b.light && b.n_units == 0) b.n_light = A.M;  // one item per strip
     const int64_t nitems = b.n_light + b.n_units;
-    const int64_t nblk = (nitems + 3) / 4;
+    const int64_t nblk = b.order ? b.n_slots / 4 : (nitems + 3) / 4;
     ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many strips");
     hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP>), dim3((unsigned)nblk), dim3(256), 0, st, A, x, y, alpha,
                        beta, b);
     ABHSF_HIP(hipGetLastError());
     if (b.n_heavy) {
-        hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)b.n_heavy), dim3(A.s < 256 ? 64 : 256), 0, st, A.s,
+        hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)b.n_heavy), dim3(kFixThreads), 0, st, A.s,
                            A.m, y, alpha, beta, b);
         ABHSF_HIP(hipGetLastError());
     }
@@ -836,15 +934,25 @@ void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int6
 
 int batch_code(int bcol_bytes, int s) { return (bcol_bytes == 4 ? 3 : 0) + (s <= 8 ? 0 : (s <= 16 ? 1 : 2)); }
 
-void batch_spmv(const BatchEntry *dE, int nmat, int64_t total_items, const uint32_t *dheavy, int64_t nheavy,
-                double alpha, double beta, int grid, hipStream_t st)
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t total_items, const uint32_t *dheavy,
+                int64_t nheavy, double alpha, double beta, int grid, hipStream_t st, unsigned long long *cycles)
 {
     if (total_items > 0) {
-        hipLaunchKernelGGL(k_batch_spmv, dim3((unsigned)grid), dim3(256), 0, st, dE, nmat, total_items, alpha, beta);
+        if (cycles)
+            hipLaunchKernelGGL(k_batch_spmv<true>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat,
+                               total_items, alpha, beta, cycles);
+        else
+            hipLaunchKernelGGL(k_batch_spmv<false>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat,
+                               total_items, alpha, beta, cycles);
         ABHSF_HIP(hipGetLastError());
     }
     if (nheavy > 0) {
-        hipLaunchKernelGGL(k_batch_fixup, dim3((unsigned)nheavy), dim3(64), 0, st, dE, dheavy, alpha, beta);
+        if (cycles)
+            hipLaunchKernelGGL(k_batch_fixup<true>, dim3((unsigned)nheavy), dim3(kFixThreads), 0, st, dE, dheavy,
+                               alpha, beta, cycles);
+        else
+            hipLaunchKernelGGL(k_batch_fixup<false>, dim3((unsigned)nheavy), dim3(kFixThreads), 0, st, dE, dheavy,
+                               alpha, beta, cycles);
         ABHSF_HIP(hipGetLastError());
     }
 }

@@ -5,16 +5,26 @@
 
 namespace abhsf {
 
-// Work decomposition of the single-RHS kernel: light strips are whole work
-// items; heavy strips are cut into units (one scheme group's block range each)
-// whose partial row sums are combined by a fix-up pass.
+// Work decomposition of the single-RHS kernel: light strips are processed
+// whole, several consecutive ones per work item when they are small; heavy
+// strips are cut into units (one scheme group's block range each) whose
+// partial row sums are combined by a fix-up pass.
 struct Balance {
-    const uint32_t *light = nullptr;  // [n_light] strip ids (null: every strip, in order)
+    const uint32_t *light = nullptr;  // light strip ids (null: every strip, in order)
+    const uint32_t *pack = nullptr;   // [n_light + 1] light item k = light[pack[k] .. pack[k+1])
+                                      // (null: one strip per light item)
     const uint32_t *units = nullptr;  // [n_units][4] {scheme, b0, b1, value offset of b0}
     const uint32_t *heavy = nullptr;  // [n_heavy][3] {strip, first unit, unit count}
     double *part = nullptr;           // [n_units][s] (x2 when compensated)
-    int64_t n_light = 0, n_units = 0, n_heavy = 0;
+    // Column-panel variant: slot table, 4 slots (waves) per workgroup; slot
+    // 4b + w holds the item of wave w of workgroup b, or kNoItem.  Workgroup b
+    // runs on XCD b % kXcdCount, so items are grouped by panel = XCD.
+    const uint32_t *order = nullptr;  // [n_slots]
+    int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0;
 };
+
+constexpr int kXcdCount = 8;            // XCDs of an MI355X; workgroups go round-robin
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 
 struct SpmvLaunch {
     int bcol_bytes;  // 2 or 4
@@ -35,9 +45,12 @@ struct BatchEntry {
 int batch_code(int bcol_bytes, int s);
 
 // All matrices' work items in one grid-stride launch, then the heavy-strip
-// fix-up (dheavy: pairs {matrix, heavy index}).  FP64 storage only.
-void batch_spmv(const BatchEntry *dE, int nmat, int64_t total_items, const uint32_t *dheavy, int64_t nheavy,
-                double alpha, double beta, int grid, hipStream_t st);
+// fix-up (dheavy: pairs {matrix, heavy index}).  item_mat[g]: matrix of
+// global item g.  FP64 storage only.  With cycles (device, [nmat], zeroed by
+// the caller) every wave adds its shader-clock cycles to its matrix's counter.
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t total_items, const uint32_t *dheavy,
+                int64_t nheavy, double alpha, double beta, int grid, hipStream_t st,
+                unsigned long long *cycles = nullptr);
 
 // Largest power-of-two rhs tile whose s x R accumulator fits the LDS budget.
 int max_rhs_tile(int s, bool comp);

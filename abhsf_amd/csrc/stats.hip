@@ -50,16 +50,41 @@ __global__ void k_sample_keys(const int64_t *__restrict__ rp, const int32_t *__r
     pay[o] = 0;
 }
 
+// Run heads measure their run (the block's entry count z) and the wave
+// aggregates: one atomic per distinct bin per wave (most blocks of sparse
+// matrices share a handful of small z, so per-lane atomics would contend).
 __global__ void k_runs_hist(const uint64_t *__restrict__ key, uint64_t n, unsigned long long *__restrict__ hist,
                             uint64_t cap, unsigned long long *__restrict__ nblocks)
 {
     const uint64_t e = (uint64_t)blockIdx.x * kT + threadIdx.x;
-    if (e >= n) return;
-    if (e > 0 && key[e] == key[e - 1]) return;  // not a run head
-    uint64_t z = 1;
-    while (e + z < n && key[e + z] == key[e]) z++;
-    atomicAdd(&hist[z < cap ? z : cap - 1], 1ull);
-    atomicAdd(nblocks, 1ull);
+    const int lane = (int)(threadIdx.x & 63);
+    uint64_t z = 0;  // 0: not a run head
+    if (e < n && (e == 0 || key[e] != key[e - 1])) {
+        // run end by galloping then bisection (keys are sorted): O(log z)
+        const uint64_t k0 = key[e];
+        uint64_t lo = 1, hi = 1;  // key[e + lo - 1] == k0; answer z in [lo, hi]
+        while (e + hi < n && key[e + hi] == k0) {
+            lo = hi + 1;
+            hi = 2 * hi;
+        }
+        if (e + hi > n) hi = n - e;
+        while (lo < hi) {  // smallest z in [lo, hi] with e + z == n or key[e + z] != k0
+            const uint64_t mid = (lo + hi) >> 1;
+            if (key[e + mid] == k0) lo = mid + 1; else hi = mid;
+        }
+        z = lo;
+        if (z >= cap) z = cap - 1;
+    }
+    const uint64_t heads = __ballot(z != 0);
+    if (lane == 0 && heads) atomicAdd(nblocks, (unsigned long long)__popcll(heads));
+    bool pending = z != 0;
+    for (uint64_t live = heads; live; live = __ballot(pending)) {
+        const int leader = __ffsll((unsigned long long)live) - 1;
+        const uint64_t zl = __shfl(z, leader, 64);
+        const uint64_t same = __ballot(pending && z == zl);
+        if (lane == leader) atomicAdd(&hist[zl], (unsigned long long)__popcll(same));
+        if (z == zl) pending = false;
+    }
 }
 
 }  // namespace

@@ -18,9 +18,10 @@ struct Storage {
     uint64_t nb[4] = {0, 0, 0, 0}, ne[4] = {0, 0, 0, 0};
     DevBuf sptr[4], eptr[4], bcol[4], z1[4], val[4];
     DevBuf coo_idx, csr_rowptr, csr_col, bm_words;
-    // work decomposition of the single-RHS kernel (heavy strips split in units)
-    DevBuf bal_light, bal_units, bal_heavy, bal_part;
-    int64_t n_light = 0, n_units = 0, n_heavy = 0;
+    // work decomposition of the single-RHS kernel (heavy strips split in units;
+    // with column panels, the slot table places each item on its panel's XCD)
+    DevBuf bal_light, bal_pack, bal_units, bal_heavy, bal_part, bal_order;
+    int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0;
     // footprints (bits) under both cost models
     uint64_t smf_preproc = 0, smf_msmf = 0;
     double build_ms = 0.0;
@@ -28,8 +29,13 @@ struct Storage {
     DevMat view() const;
     SpmvLaunch launch() const;
     // Split strips whose work exceeds 2*unit entries into units of about
-    // `unit` entries (unit <= 0: no splitting).
-    void build_balance(int64_t unit, hipStream_t st);
+    // `unit` entries, and pack runs of consecutive light strips into items of
+    // at most unit/2 entries (unit <= 0: neither).  With panels > 1 the block
+    // columns are also cut into `panels` column panels of equal work: a strip
+    // whose blocks span several panels is split at the panel boundaries, and
+    // every work item is placed on the XCD of its panel (panels must equal the
+    // XCD count, kXcdCount).
+    void build_balance(int64_t unit, hipStream_t st, int panels = 1);
     uint64_t stored_bytes() const;
     uint64_t value_bytes() const;
     void set_shape(int64_t m_, int64_t n_, int s_);

@@ -51,7 +51,8 @@ def _addr(a) -> int:
 
 _POLICY = {"preproc": capi.POLICY_PREPROC, "msmf": capi.POLICY_MSMF}
 _PREC = {"f64": capi.F64, "f32": capi.F32}
-_VAR = {"auto": capi.VARIANT_AUTO, "strip": capi.VARIANT_STRIP, "balanced": capi.VARIANT_BALANCED}
+_VAR = {"auto": capi.VARIANT_AUTO, "strip": capi.VARIANT_STRIP, "balanced": capi.VARIANT_BALANCED,
+        "panel": capi.VARIANT_PANEL}
 
 
 class AbhsfMatrix:
@@ -215,6 +216,13 @@ class AbhsfBatch:
 
     def synchronize(self) -> None:
         capi.call("abhsf_batch_synchronize", self._h)
+
+    def profile_cycles(self, alpha: float = 1.0, beta: float = 0.0) -> np.ndarray:
+        """One eager batched SpMV returning, per member, the shader-clock
+        cycles its work items occupied (its share of the launch)."""
+        out = np.zeros(len(self.mats), dtype=np.uint64)
+        capi.call("abhsf_batch_profile", self._h, float(alpha), float(beta), out.ctypes.data_as(C.c_void_p))
+        return out
 
     def release(self) -> None:
         if self._h and self._h.value:

@@ -138,6 +138,25 @@ def uniform_blocks(nb: int, bs: int, bnnz: int, seed: int = 0x0498BC04) -> Csr:
     return Csr(n, n, rowptr, col, val, f"uniform_blocks_{nb}x{bs}_{bnnz}")
 
 
+def batch_member(rng, i: int, seed: int) -> Csr:
+    """One matrix of the multi-matrix batch workload: n log-uniform in
+    [1e3, 2e5], family drawn uniformly from 5-point Laplacian / banded
+    (bandwidth 4-64) / R-MAT edge factor 8 / random blocks (4-32, 2-8 blocks
+    per block row, density 1-30%); rng draws the parameters, seed + i seeds
+    the generator."""
+    import math
+    n = int(math.exp(rng.uniform(math.log(1e3), math.log(2e5))))
+    fam = int(rng.integers(0, 4))
+    if fam == 0:
+        return laplacian2d(max(2, int(round(math.sqrt(n)))))
+    if fam == 1:
+        return banded(n, int(rng.integers(4, 65)), seed=seed + i)
+    if fam == 2:
+        return rmat(max(4, int(round(math.log2(n)))), 8, seed=seed + i)
+    return random_blocks(n, int(rng.choice([4, 8, 16, 32])), int(rng.integers(2, 9)),
+                         float(rng.uniform(0.01, 0.30)), seed=seed + i)
+
+
 def from_triplets(m: int, n: int, rows, cols, vals, name: str = "") -> Csr:
     rows = np.asarray(rows, dtype=np.int64)
     cols = np.asarray(cols, dtype=np.int64)

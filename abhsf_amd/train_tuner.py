@@ -1,6 +1,8 @@
 """Train the learned tuner on the GPU and judge it against exhaustive search.
 
-  python -m abhsf_amd.train_tuner --out DIR [--steps 4000]
+  python -m abhsf_amd.train_tuner --out DIR [--steps 4000] [--mode latency|batch]
+
+Mode latency (single-matrix SpMV):
 
 1. corpus: seeded synthetic matrices of every family (train and held-out
    sets use different seeds and sizes; the held-out set contains the R-MAT
@@ -13,6 +15,16 @@
    (s = 64) and of the storage-optimal block size (the reference's search,
    block_statistics/mmf.h:77-102), all against the exhaustive best.
 Writes DIR/tuner_model.json, DIR/tuner_dataset.npz and DIR/tuner_report.json.
+
+Mode batch (many matrices in one launch): matrices drawn like the batch
+workload (synth.batch_member, other seeds: 300 train, 100 held-out); for each
+candidate every member is built at that candidate and one batched launch is
+profiled (abhsf_batch_profile): the label is log(cycles per entry) of the
+member's work items inside the full launch, i.e. its share of the GPU.  The
+held-out report gives per-matrix regret and the measured batched launch time
+of the model's picks against plain s = 64 and the best single candidate.
+Writes DIR/tuner_model_batch.json, DIR/tuner_dataset_batch.npz and
+DIR/tuner_report_batch.json.
 """
 from __future__ import annotations
 
@@ -78,16 +90,128 @@ def collect(split, stream, log):
     return np.array(X), np.array(y), names, times
 
 
+def batch_corpus(split):
+    seed, count = (1000, 300) if split == "train" else (2000, 100)
+    rng = np.random.default_rng(seed)
+    return [synth.batch_member(rng, i, seed) for i in range(count)]
+
+
+def time_graph(B, stream, reps=5):
+    for _ in range(2):
+        B.spmv_async(1.0, 1.0, graph=True)
+    B.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        B.spmv_async(1.0, 1.0, graph=True)
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def build_batch(mats, picks, xs, ys, stream):
+    from .matrix import AbhsfBatch, AbhsfMatrix
+    Ms = []
+    for A, (s, v) in zip(mats, picks):
+        M = AbhsfMatrix(block_size=s, variant=v, stream=stream)
+        M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+        Ms.append(M)
+    B = AbhsfBatch(Ms, stream)
+    B.set_vectors(xs, ys)
+    return B, Ms
+
+
+def collect_batch(split, stream, log):
+    """Features and in-launch cycle labels of every member for every candidate."""
+    t0 = time.perf_counter()
+    mats = batch_corpus(split)
+    cands = tuner.candidates()
+    X = np.zeros((len(mats), len(cands), tuner.NFEAT))
+    for i, A in enumerate(mats):
+        X[i] = tuner.matrix_features(A)[0]
+    log(f"{split}: {len(mats)} matrices, {sum(A.nnz for A in mats)} nnz, features "
+        f"({time.perf_counter() - t0:.0f} s)")
+    xs = [hip.DeviceArray.from_numpy(np.random.default_rng(i).uniform(-1, 1, A.n)) for i, A in enumerate(mats)]
+    ys = [hip.DeviceArray.zeros(A.m) for A in mats]
+    cyc = np.zeros((len(mats), len(cands)))
+    graph_ms = {}
+    for c, cand in enumerate(cands):
+        B, Ms = build_batch(mats, [cand] * len(mats), xs, ys, stream)
+        B.profile_cycles()  # warm-up
+        runs = np.stack([B.profile_cycles().astype(np.float64) for _ in range(3)])
+        cyc[:, c] = np.median(runs, axis=0)
+        graph_ms[cand] = time_graph(B, stream)
+        B.release()
+        for M in Ms:
+            M.release()
+        log(f"{split} {cand}: batched launch {graph_ms[cand]:.3f} ms, sum of cycles {cyc[:, c].sum():.3e}")
+    nnz = np.array([A.nnz for A in mats], dtype=np.float64)
+    return mats, X, cyc, nnz, graph_ms, xs, ys
+
+
+def main_batch(args, log):
+    stream = hip.Stream()
+    cands = tuner.candidates()
+    _, Xtr, ctr, ntr, gtr, _, _ = collect_batch("train", stream, log)
+    Xf = Xtr.reshape(-1, tuner.NFEAT)
+    ytr = np.log(np.maximum(ctr, 1.0) / ntr[:, None]).reshape(-1)
+    w = np.repeat(np.sqrt(ntr / ntr.mean()), len(cands))
+    model, losses = tuner.train(Xf, ytr, w=w, steps=args.steps, lr=args.lr)
+    log(f"trained: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {model.meta['train_s']:.1f} s")
+    model.meta["labels"] = "log(cycles per entry inside a batched launch)"
+    model.save(os.path.join(args.out, "tuner_model_batch.json"))
+    mats, Xte, cte, nte, gte, xs, ys = collect_batch("test", stream, log)
+    np.savez(os.path.join(args.out, "tuner_dataset_batch.npz"), Xtr=Xtr, ctr=ctr, ntr=ntr, Xte=Xte, cte=cte,
+             nte=nte)
+    picks, regrets, reg64 = [], [], []
+    i64 = cands.index((64, "balanced"))
+    for i in range(len(mats)):
+        pred = model.predict(Xte[i])
+        k = int(np.argmin(pred))
+        picks.append(cands[k])
+        best = cte[i].min()
+        regrets.append((cte[i, k] - best) / best * 100.0)
+        reg64.append((cte[i, i64] - best) / best * 100.0)
+    B, Ms = build_batch(mats, picks, xs, ys, stream)
+    ms_pick = time_graph(B, stream)
+    B.release()
+    best_fixed = min(gte, key=gte.get)
+    oracle_cyc = cte.min(axis=1).sum()
+    report = {
+        "labels": "cycles of the member's work items inside one batched launch (abhsf_batch_profile)",
+        "train_matrices": int(len(ntr)), "test_matrices": int(len(nte)),
+        "train_loss_first": float(losses[0]), "train_loss_last": float(losses[-1]),
+        "mean_regret_pct": {"mlp": float(np.mean(regrets)), "plain_s64": float(np.mean(reg64))},
+        "sum_cycles_vs_per_matrix_best": {
+            "mlp": float(sum(cte[i, cands.index(p)] for i, p in enumerate(picks)) / oracle_cyc),
+            "plain_s64": float(cte[:, i64].sum() / oracle_cyc),
+            "best_fixed": float(cte[:, cands.index(best_fixed)].sum() / oracle_cyc)},
+        "batched_launch_ms": {"mlp_picks": ms_pick, "plain_s64": gte[(64, "balanced")],
+                              "best_fixed": gte[best_fixed], "best_fixed_candidate": list(best_fixed)},
+        "speedup_vs_plain_s64": gte[(64, "balanced")] / ms_pick,
+        "picks": {f"{s}/{v}": sum(1 for p in picks if p == (s, v)) for s, v in cands},
+        "per_candidate_launch_ms": {f"{s}/{v}": gte[(s, v)] for s, v in cands},
+    }
+    json.dump(report, open(os.path.join(args.out, "tuner_report_batch.json"), "w"), indent=1)
+    log(f"held-out batch: MLP picks {ms_pick:.3f} ms, plain s64 {gte[(64, 'balanced')]:.3f} ms "
+        f"({report['speedup_vs_plain_s64']:.2f}x), best fixed {best_fixed} {gte[best_fixed]:.3f} ms; "
+        f"mean regret {report['mean_regret_pct']}")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--mode", choices=["latency", "batch"], default="latency")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
 
     def log(msg):
         print(f"[train_tuner] {msg}", flush=True)
+
+    if args.mode == "batch":
+        return main_batch(args, log)
 
     stream = hip.Stream()
     Xtr, ytr, ntr, ttr = collect("train", stream, log)

@@ -3,7 +3,10 @@
 A 2 x 64 hidden-layer FP32 MLP (abhsf_amd/csrc/mlp.hip, trained and run on
 the GPU through the C ABI) predicts log(SpMV time per entry) for every
 candidate (s, variant) from cheap matrix statistics; the tuner picks the
-candidate with the smallest prediction.  Labels come from exhaustive timing
+candidate with the smallest prediction.  Two models share the features: one
+trained on single-launch latency (MODEL_PATH), one on per-matrix throughput
+inside a batched launch (BATCH_MODEL_PATH), where launch-level parallelism
+comes from the other matrices and per-item overheads dominate.  Labels come from exhaustive timing
 sweeps of this library (`sweep`), which also judge the tuner's regret
 (block_statistics/consistency.cpp:147-149: (t_pick - t_min) / t_min * 100).
 
@@ -15,7 +18,7 @@ Features (the statistics block_statistics/ computes, block_stats.cpp:78-100,
               entries in each scheme, log2 predicted stored bytes per entry,
               strip-work stdev / mean, log2 (max strip work / mean),
               fraction of entries in strips above the split threshold,
-              log2 strips, variant flag
+              log2 strips, variant flags (balanced, panel)
 Block-size histograms come from abhsf_block_hist on a sample of strips.
 """
 from __future__ import annotations
@@ -32,9 +35,12 @@ from . import capi, hip
 
 NAME = "mlp-2x64"
 S_CANDIDATES = (8, 16, 32, 64, 128, 256)
-VARIANTS = ("strip", "balanced")
+VARIANTS = ("strip", "balanced", "panel")
 HIDDEN = 64
-MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_model.json")
+_HERE = os.path.dirname(os.path.abspath(__file__))
+MODEL_PATH = os.path.join(_HERE, "tuner_model.json")          # single-matrix SpMV latency
+BATCH_MODEL_PATH = os.path.join(_HERE, "tuner_model_batch.json")  # throughput inside a batched launch
+BATCH_TARGET_NNZ = 30_000_000  # entries per timed batch of replicas
 SAMPLE_NNZ = 2_000_000  # entries counted per block-size histogram
 
 
@@ -146,11 +152,11 @@ def feature_vector(g, sf, variant):
         g["log2_n"], g["log2_nnz"], g["log2_mean_row"], g["row_cv"], g["log2_max_row"], g["empty_rows"],
         sf["log2_s"], sf["log2_blk_per_entry"], *sf["blk_frac"], *sf["ent_frac"],
         sf["log2_bytes_per_entry"], sf["strip_cv"], sf["log2_max_strip"], sf["heavy_frac"], sf["log2_M"],
-        1.0 if variant == "balanced" else 0.0,
+        1.0 if variant == "balanced" else 0.0, 1.0 if variant == "panel" else 0.0,
     ], dtype=np.float64)
 
 
-NFEAT = 6 + 15
+NFEAT = 6 + 16
 
 
 def candidates():
@@ -196,11 +202,44 @@ def time_candidate(A, s, variant, d_rp, d_c, d_v, x, y, stream, reps=10):
     return float(np.median(times))
 
 
-def sweep(A, stream=None):
-    """Time every candidate on A; returns {(s, variant): ms}."""
+def time_candidate_batched(A, s, variant, d_rp, d_c, d_v, x, stream, reps=5):
+    """Throughput label: ms per copy of A inside one batched launch (hipGraph
+    replay) of R references to the same built matrix, R chosen so that the
+    launch holds about BATCH_TARGET_NNZ entries.  The copies share x and the
+    heavy-strip partial buffer (so their y values are not meaningful: this
+    is a timing run); each has its own y."""
+    from .matrix import AbhsfBatch, AbhsfMatrix
+    M = AbhsfMatrix(block_size=s, variant=variant, stream=stream)
+    M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+    R = int(min(256, max(1, -(-BATCH_TARGET_NNZ // max(A.nnz, 1)))))
+    ys = [hip.DeviceArray.zeros(A.m) for _ in range(R)]
+    B = AbhsfBatch([M] * R, stream)
+    B.set_vectors([x] * R, ys)
+    for _ in range(2):
+        B.spmv_async(1.0, 1.0, graph=True)
+    B.synchronize()
+    times = []
+    for _ in range(3):
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            B.spmv_async(1.0, 1.0, graph=True)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1) / reps / R)
+    B.release()
+    M.release()
+    return float(np.median(times))
+
+
+def sweep(A, stream=None, batched=False):
+    """Time every candidate on A; returns {(s, variant): ms} (single launch
+    latency, or per-matrix throughput inside a batch when batched)."""
     stream = stream or hip.Stream()
     d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
     x = hip.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, A.n))
+    if batched:
+        return {(s, v): time_candidate_batched(A, s, v, d_rp, d_c, d_v, x, stream) for s, v in candidates()}
     y = hip.DeviceArray.zeros(A.m)
     return {(s, v): time_candidate(A, s, v, d_rp, d_c, d_v, x, y, stream) for s, v in candidates()}
 

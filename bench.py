@@ -151,9 +151,75 @@ def tuned_pick(A, allowed_s, hip):
     return cands[best], tuner.NAME
 
 
-def roofline(r):
+def measured_traffic(key):
+    """HBM bytes per launch of the SpMV kernel for this configuration, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
+    profiles/pmc_traffic.json (each counter scaled by what it reads for the
+    plain copy kernel of known size); None when that configuration was not
+    profiled."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get(key)
+    return None if rec is None else int(rec["traffic_bytes"])
+
+
+def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
+    """Per-dispatch HBM traffic of the SpMV kernels from rocprofv3 --pmc CSVs.
+
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are averaged over the
+    k_spmv_strip dispatches (plus k_spmv_fixup when present) and scaled by what
+    the same counter reads for k_stream_copy, the calibration copy of known
+    size (2 GiB read + 2 GiB written per dispatch, calib_copy_gbs above).  With
+    sizes_csv (a pass over TCC_EA0_RDREQ_{32B,64B,128B}_sum and
+    TCC_EA0_RDREQ_sum) the read bytes are also resolved by request size.
+    Writes the entry into profiles/pmc_traffic.json."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(path):
+        acc = defaultdict(lambda: defaultdict(list))
+        for row in csv.DictReader(open(path)):
+            nm = row["Kernel_Name"]
+            kind = ("spmv" if "k_spmv_strip" in nm else "fixup" if "k_spmv_fixup" in nm
+                    else "copy" if "k_stream_copy" in nm else None)
+            if kind:
+                acc[kind][row["Counter_Name"]].append(float(row["Counter_Value"]))
+        return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
+
+    copy_bytes = 2 << 30
+    f, w = per_kernel(fetch_csv), per_kernel(write_csv)
+    f_factor = copy_bytes / (f["copy"]["FETCH_SIZE"] * 1024)
+    w_factor = copy_bytes / (w["copy"]["WRITE_SIZE"] * 1024)
+    fetch = sum(f[k]["FETCH_SIZE"] for k in ("spmv", "fixup") if k in f) * 1024
+    write = sum(w[k]["WRITE_SIZE"] for k in ("spmv", "fixup") if k in w) * 1024
+    rec = {"fetch_kib": round(fetch / 1024, 1), "write_kib": round(write / 1024, 1),
+           "fetch_factor": round(f_factor, 4), "write_factor": round(w_factor, 4),
+           "traffic_bytes": int(fetch * f_factor + write * w_factor), "command": command}
+    if sizes_csv:
+        z = per_kernel(sizes_csv)
+
+        def by_size(d):
+            return (32 * d.get("TCC_EA0_RDREQ_32B_sum", 0) + 64 * d.get("TCC_EA0_RDREQ_64B_sum", 0)
+                    + 128 * d.get("TCC_EA0_RDREQ_128B_sum", 0))
+        rec["read_bytes_by_request_size"] = int(sum(by_size(z[k]) for k in ("spmv", "fixup") if k in z))
+        rec["copy_read_bytes_by_request_size"] = int(by_size(z["copy"]))
+        rec["read_requests"] = {c: int(sum(z[k].get(c, 0) for k in ("spmv", "fixup") if k in z))
+                                for c in ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum",
+                                          "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")}
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    allrec = json.load(open(path)) if os.path.exists(path) else {}
+    allrec[key] = rec
+    with open(path, "w") as fh:
+        json.dump(allrec, fh, indent=1, sort_keys=True)
+        fh.write("\n")
+    return rec
+
+
+def roofline(r, key=None):
     return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None}
+            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(key) if key else None,
+            "algorithmic_bytes": r.get("algo_bytes")}
 
 
 def base_record(args, value, ms, dtype="f64"):
@@ -168,7 +234,7 @@ def run_block_mix(args, hip):
     A = synth.block_mix(args.n, seed=0)
     log(f"matrix block_mix n={A.n} nnz={A.nnz} checksum={A.checksum():#x} ({time.perf_counter() - t0:.1f} s)")
     stream = hip.Stream()
-    s_list = [int(v) for v in args.s.split(",")]
+    s_list = [int(v) for v in (args.s or "16,32,64,128").split(",")]
     pick, tname = tuned_pick(A, s_list, hip)
     cands = [(s, "balanced") for s in s_list]
     if pick is not None and pick not in cands:
@@ -185,7 +251,7 @@ def run_block_mix(args, hip):
                     f"50% 12/row), N(0,1) FP64 values, seed 0",
         "nnz": A.nnz, "s": pick[0], "variant": pick[1], "s_measured": s_list, "tuner": tname,
     }
-    out["roofline"] = roofline(best)
+    out["roofline"] = roofline(best, f"block_mix/{pick[0]}/{pick[1]}")
     out["plain_s64"] = None if plain is None else {"gflops": plain["gflops"], "gbs": plain["gbs"]}
     out["tuned_vs_plain"] = None if plain is None else round(best["gflops"] / plain["gflops"], 3)
     out["exhaustive_best"] = {"s": max(res, key=lambda k: res[k]["gflops"])[0],
@@ -204,22 +270,27 @@ def run_rmat(args, hip):
     A = synth.rmat(22, 16, seed=0)
     log(f"matrix rmat scale 22 ef 16 nnz={A.nnz}")
     stream = hip.Stream()
-    pick, tname = tuned_pick(A, tuner.S_CANDIDATES, hip)
-    cands = tuner.candidates()
+    s_allowed = tuner.S_CANDIDATES if args.s is None else [int(v) for v in args.s.split(",")]
+    pick, tname = tuned_pick(A, s_allowed, hip)
+    cands = [c for c in tuner.candidates() if c[0] in s_allowed]
+    if args.s is not None:  # a restricted run (profiling): the balanced variant only, plus the pick
+        cands = [c for c in cands if c[1] == "balanced" or c == pick]
     res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
     if pick is None:
         pick, tname = max(res, key=lambda k: res[k]["gflops"]), "exhaustive-timing"
-    best, plain = res[pick], res[(64, "balanced")]
+    best, plain = res[pick], res.get((64, "balanced"), res[pick])
     times = {k: r["ms"] for k, r in res.items()}
     out = base_record(args, best["gflops"], best["ms"])
     out["config"] = {"workload": "R-MAT scale 22 (a=.57,b=.19,c=.19), edge factor 16, duplicates merged, "
                      "U[-1,1] FP64 values, seed 0", "nnz": A.nnz, "s": pick[0], "variant": pick[1],
                      "tuner": tname}
-    out["roofline"] = roofline(best)
+    out["roofline"] = roofline(best, f"rmat/{pick[0]}/{pick[1]}")
     out["plain_s64"] = {"gflops": plain["gflops"], "gbs": plain["gbs"]}
     out["tuned_vs_plain"] = round(best["gflops"] / plain["gflops"], 3)
     out["regret_pct"] = round(tuner.regret(times, pick), 2)
     out["per_candidate"] = {f"{s}/{v}": r for (s, v), r in res.items()}
+    if not args.no_calib:
+        out["copy_calibration"] = calib_copy_gbs(hip)
     return out
 
 
@@ -273,39 +344,35 @@ def run_spmm(args, hip):
 
 
 def run_batch(args, hip):
+    """Many matrices in one launch: the tuner's (s, variant) per matrix against
+    plain ABHSF at s = 64 for all, both as one batched hipGraph replay."""
     from abhsf_amd import AbhsfBatch, AbhsfMatrix, synth, tuner
     rng = np.random.default_rng(0)
     stream = hip.Stream()
-    model = tuner.Model.load() if os.path.exists(tuner.MODEL_PATH) else None
-    mats, xs, ys, nnz, indiv = [], [], [], 0, []
+    mpath = tuner.BATCH_MODEL_PATH if os.path.exists(tuner.BATCH_MODEL_PATH) else tuner.MODEL_PATH
+    model = tuner.Model.load(mpath) if os.path.exists(mpath) else None
+    tuned, plain, xs, ys, nnz, indiv, picks = [], [], [], [], 0, [], {}
     t0 = time.perf_counter()
     for i in range(args.batch_count):
-        n = int(math.exp(rng.uniform(math.log(1e3), math.log(2e5))))
-        fam = int(rng.integers(0, 4))
-        if fam == 0:
-            A = synth.laplacian2d(max(2, int(round(math.sqrt(n)))))
-        elif fam == 1:
-            A = synth.banded(n, int(rng.integers(4, 65)), seed=i)
-        elif fam == 2:
-            A = synth.rmat(max(4, int(round(math.log2(n)))), 8, seed=i)
-        else:
-            A = synth.random_blocks(n, int(rng.choice([4, 8, 16, 32])), int(rng.integers(2, 9)),
-                                    float(rng.uniform(0.01, 0.30)), seed=i)
+        A = synth.batch_member(rng, i, 0)
         s, v = 64, "balanced"
         if model is not None:
             (s, v), _ = tuner.pick(model, tuner.matrix_features(A)[0])
-        M = AbhsfMatrix(block_size=s, variant=v, stream=stream)
-        M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
-        x = hip.DeviceArray.from_numpy(np.random.default_rng(i).uniform(-1, 1, A.n))
-        y = hip.DeviceArray.zeros(A.m)
-        mats.append(M)
-        xs.append(x)
-        ys.append(y)
+        picks[f"{s}/{v}"] = picks.get(f"{s}/{v}", 0) + 1
+        d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
+        for lst, (bs, var) in ((tuned, (s, v)), (plain, (64, "balanced"))):
+            M = AbhsfMatrix(block_size=bs, variant=var, stream=stream)
+            M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+            lst.append(M)
+        stream.synchronize()
+        del d_rp, d_c, d_v
+        xs.append(hip.DeviceArray.from_numpy(np.random.default_rng(i).uniform(-1, 1, A.n)))
+        ys.append(hip.DeviceArray.zeros(A.m))
         nnz += A.nnz
         if i % 200 == 0:
             log(f"built {i} matrices, {nnz} nnz ({time.perf_counter() - t0:.0f} s)")
-    # per-matrix latency: each matrix alone
-    for M, x, y in zip(mats, xs, ys):
+    # per-matrix latency: each tuned matrix alone
+    for M, x, y in zip(tuned, xs, ys):
         M.spmv_async(1.0, x, 1.0, y)
         stream.synchronize()
         e0, e1 = hip.Event(), hip.Event()
@@ -314,22 +381,33 @@ def run_batch(args, hip):
         e1.record(stream)
         e1.synchronize()
         indiv.append(e0.elapsed_ms(e1))
-    B = AbhsfBatch(mats, stream)
-    B.set_vectors(xs, ys)
-    for _ in range(args.warmup):
-        B.spmv_async(1.0, 1.0, graph=True)
-    B.synchronize()
-    e0, e1 = hip.Event(), hip.Event()
-    e0.record(stream)
-    for _ in range(args.steps):
-        B.spmv_async(1.0, 1.0, graph=True)
-    e1.record(stream)
-    e1.synchronize()
-    ms = e0.elapsed_ms(e1) / args.steps
+
+    def time_batch(mats):
+        B = AbhsfBatch(mats, stream)
+        B.set_vectors(xs, ys)
+        for _ in range(args.warmup):
+            B.spmv_async(1.0, 1.0, graph=True)
+        B.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(args.steps):
+            B.spmv_async(1.0, 1.0, graph=True)
+        e1.record(stream)
+        e1.synchronize()
+        B.release()
+        return e0.elapsed_ms(e1) / args.steps
+
+    ms = time_batch(tuned)
+    ms_plain = time_batch(plain)
+    log(f"batched: tuned {ms:.3f} ms, plain s=64 {ms_plain:.3f} ms")
     out = base_record(args, round(2 * nnz / (ms * 1e-3) / 1e9, 2), round(ms, 4))
     out["config"] = {"workload": f"{args.batch_count} matrices, n log-uniform [1e3, 2e5], families Laplacian / "
                      "banded bw 4-64 / R-MAT ef 8 / random blocks 1-30%, N(0,1) FP64, one hipGraph launch",
-                     "nnz": nnz, "tuner": tuner.NAME if model is not None else "fixed s=64"}
+                     "nnz": nnz, "picks": picks,
+                     "tuner": (tuner.NAME + (" (throughput model)" if mpath == tuner.BATCH_MODEL_PATH else
+                                             " (latency model)")) if model is not None else "fixed s=64"}
+    out["plain_s64"] = {"gflops": round(2 * nnz / (ms_plain * 1e-3) / 1e9, 2), "ms": round(ms_plain, 4)}
+    out["tuned_vs_plain"] = round(ms_plain / ms, 3)
     out["per_matrix_latency_ms"] = {"p50": round(float(np.percentile(indiv, 50)), 4),
                                     "p99": round(float(np.percentile(indiv, 99)), 4),
                                     "sum_individual": round(float(np.sum(indiv)), 3)}
@@ -343,14 +421,25 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="block_mix", choices=["block_mix", "rmat", "spmm", "batch"])
-    ap.add_argument("--s", default="16,32,64,128")
+    ap.add_argument("--s", default=None, help="block sizes to measure (block_mix default 16,32,64,128; "
+                    "rmat default all tuner candidates)")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--spmm-s", type=int, default=64)
     ap.add_argument("--batch-count", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-calib", action="store_true")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="ARG",
+                    help="KEY FETCH_CSV WRITE_CSV [SIZES_CSV]: summarise rocprofv3 --pmc counter CSVs of a "
+                         "profiled bench run into profiles/pmc_traffic.json and exit (no GPU needed)")
+    ap.add_argument("--pmc-command", default="", help="the profiled command, recorded with --pmc-summary")
     args = ap.parse_args()
+    if args.pmc_summary:
+        a = args.pmc_summary
+        if len(a) not in (3, 4):
+            ap.error("--pmc-summary takes KEY FETCH_CSV WRITE_CSV [SIZES_CSV]")
+        print(json.dumps(summarize_pmc(a[0], a[1], a[2], a[3] if len(a) == 4 else None, args.pmc_command)))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
               "(the SpMV shards by block-row strips; only the single-GPU kernel is built)", file=sys.stderr)

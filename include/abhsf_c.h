@@ -67,7 +67,10 @@ enum { ABHSF_F64 = 0, ABHSF_F32 = 1 };
 enum {
     ABHSF_VARIANT_AUTO = 0,
     ABHSF_VARIANT_STRIP = 1,    /* one wavefront per strip */
-    ABHSF_VARIANT_BALANCED = 2  /* heavy strips split over several wavefronts */
+    ABHSF_VARIANT_BALANCED = 2, /* heavy strips split over several wavefronts */
+    ABHSF_VARIANT_PANEL = 3     /* balanced, plus 8 column panels of equal work, each
+                                   processed on one XCD so that its slice of x stays in
+                                   that XCD's L2; strips spanning panels are split there */
 };
 
 typedef struct abhsf_options {
@@ -175,6 +178,13 @@ abhsf_status abhsf_batch_set_vectors(abhsf_batch *b, const double *const *xs, do
 /* y_i = alpha * A_i * x_i + beta * y_i for every member, enqueued on the batch stream. */
 abhsf_status abhsf_batch_spmv(abhsf_batch *b, double alpha, double beta, int32_t use_graph);
 abhsf_status abhsf_batch_synchronize(abhsf_batch *b);
+/* One eager batched SpMV that also measures, per member matrix, the
+ * shader-clock cycles its work items occupied (sum over its wavefronts;
+ * fix-up workgroups count once per wavefront).  cycles: host uint64[count].
+ * Inside a full launch this is the matrix's share of the GPU, the label the
+ * throughput tuner learns.  No counterpart in the reference (its tuner works
+ * on storage footprints, block_statistics/mmf.h:77-102). */
+abhsf_status abhsf_batch_profile(abhsf_batch *b, double alpha, double beta, uint64_t *cycles);
 abhsf_status abhsf_batch_destroy(abhsf_batch *b);
 
 /* ---- Tuner cost model: 2 x 64 hidden ReLU MLP, FP32, on the GPU ----------

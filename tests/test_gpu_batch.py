@@ -41,7 +41,7 @@ def test_batched_launch_eager_and_graph(gpu):
     st = hip.Stream()
     Ms, xs, ys, refs, y0s = [], [], [], [], []
     for i, (A, s) in enumerate(members):
-        M = AbhsfMatrix(block_size=s, stream=st)
+        M = AbhsfMatrix(block_size=s, variant=("balanced", "strip", "panel")[i % 3], stream=st)
         M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
         x = np.random.default_rng(i).uniform(-1, 1, A.n)
         y0 = np.random.default_rng(i + 100).uniform(-1, 1, A.m)
@@ -68,4 +68,39 @@ def test_batched_launch_eager_and_graph(gpu):
         for (A, x), y, y0 in zip(refs, ys, y0s):
             ref = y0 + (rep + 1) * pyoracle.csr_spmv(A.rowptr, A.col, A.val, x, 0.5, 0.0)
             assert relerr(y.numpy(), ref) <= 1e-11, rep
+    B.release()
+
+
+def test_batch_profile_cycles(gpu):
+    """The timed launch computes the same y as the plain one and gives every
+    member a positive cycle count; heavier members cost more."""
+    members = make_members()[:8]
+    st = hip.Stream()
+    Ms, xs, ys, refs = [], [], [], []
+    for i, (A, s) in enumerate(members):
+        M = AbhsfMatrix(block_size=s, stream=st)
+        M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+        x = np.random.default_rng(i).uniform(-1, 1, A.n)
+        Ms.append(M)
+        xs.append(DeviceArray.from_numpy(x))
+        ys.append(DeviceArray.zeros(A.m))
+        refs.append((A, x))
+    B = AbhsfBatch(Ms, st)
+    B.set_vectors(xs, ys)
+    cyc = B.profile_cycles(1.0, 0.0)
+    assert cyc.shape == (len(Ms),) and np.all(cyc > 0)
+    for (A, x), y in zip(refs, ys):
+        assert relerr(y.numpy(), pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)) <= TOL
+    # a member ten times larger costs more cycles than a tiny one
+    big = synth.banded(200000, 32, 1)
+    small = synth.banded(2000, 4, 1)
+    Mb, Ms2 = AbhsfMatrix(block_size=64, stream=st), AbhsfMatrix(block_size=64, stream=st)
+    Mb.from_csr(big.m, big.n, big.rowptr, big.col, big.val)
+    Ms2.from_csr(small.m, small.n, small.rowptr, small.col, small.val)
+    B2 = AbhsfBatch([Mb, Ms2], st)
+    B2.set_vectors([DeviceArray.zeros(big.n), DeviceArray.zeros(small.n)],
+                   [DeviceArray.zeros(big.m), DeviceArray.zeros(small.m)])
+    c2 = B2.profile_cycles()
+    assert c2[0] > 10 * c2[1], c2
+    B2.release()
     B.release()

@@ -253,7 +253,7 @@ def test_device_csr_input_and_determinism(gpu):
     print(f"bitwise-identical repeated SpMV: {same}")
 
 
-@pytest.mark.parametrize("variant", ["strip", "balanced"])
+@pytest.mark.parametrize("variant", ["strip", "balanced", "panel"])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("s", [8, 64, 256])
 def test_heavy_strip_variants(gpu, variant, precision, s):
@@ -272,3 +272,42 @@ def test_heavy_strip_variants(gpu, variant, precision, s):
     vals = A.val if precision == "f64" else A.val.astype(np.float32).astype(np.float64)
     ref = pyoracle.csr_spmv(A.rowptr, A.col, vals, x, 2.0, 0.5, y0)
     assert relerr(dy.numpy(), ref) <= TOL
+
+
+PANEL_MATS = {
+    "scatter_2^15": lambda: synth.block_mix(1 << 15, 2, 0.0, 0.0),
+    "block_mix_2^15": lambda: synth.block_mix(1 << 15, 3),
+    "rmat_14": lambda: synth.rmat(14, 8, seed=9),
+    "laplacian_200": lambda: synth.laplacian2d(200),
+    "banded_5000_40": lambda: synth.banded(5000, 40, 2),
+    "narrow_3000x20": lambda: synth.random_sparse(3000, 20, 0.2, seed=5),
+    "one_row": lambda: synth.random_sparse(1, 5000, 0.01, seed=6),
+    "empty_rows": lambda: synth.from_triplets(700, 900, [0, 5, 699], [3, 899, 0], [1.0, -2.0, 3.0]),
+    "all_empty": lambda: synth.from_triplets(300, 300, [], [], []),
+}
+
+
+@pytest.mark.parametrize("name", list(PANEL_MATS))
+@pytest.mark.parametrize("s", [16, 64, 256])
+def test_panel_variant(gpu, name, s):
+    """Column-panel variant: strips cut at the 8 panel boundaries, items
+    placed per XCD by the slot table, split strips summed by the fix-up."""
+    A = PANEL_MATS[name]()
+    print(f"case panel {name} s={s}", flush=True)
+    M = AbhsfMatrix(block_size=s, variant="panel")
+    M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    check_arrays(M, A, s)
+    x = np.random.default_rng(8).uniform(-1, 1, A.n)
+    y0 = np.random.default_rng(9).uniform(-1, 1, A.m)
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
+        got, ref = spmv_check(M, A, x, y0, alpha, beta)
+        assert relerr(got, ref) <= TOL, (alpha, beta)
+    # SpMM through the same handle (its kernel does not use the panel items)
+    X = np.random.default_rng(10).uniform(-1, 1, (A.n, 3))
+    dX, dY = DeviceArray.from_numpy(X), DeviceArray.zeros((A.m, 3))
+    M.spmm_async(3, 1.0, dX, 3, 0.0, dY, 3)
+    M.synchronize()
+    Y = dY.numpy()
+    for j in range(3):
+        ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, X[:, j].copy())
+        assert relerr(Y[:, j], ref) <= TOL, j

@@ -115,6 +115,51 @@ def test_trained_model_beats_plain_abhsf_on_heldout(gpu):
     assert np.mean(r_mlp) < np.mean(r_plain)
 
 
+def test_throughput_model_beats_plain_abhsf_in_batched_launch(gpu):
+    """The throughput model (labels: cycles inside a batched launch) picks
+    per-matrix (s, variant) for a batch of unseen matrices (seeds outside
+    its training corpus); one batched launch of those picks beats the same
+    batch at plain s = 64, and the results stay exact."""
+    from abhsf_amd import AbhsfBatch, AbhsfMatrix
+    assert os.path.exists(tuner.BATCH_MODEL_PATH), "train with python -m abhsf_amd.train_tuner --mode batch"
+    model = tuner.Model.load(tuner.BATCH_MODEL_PATH)
+    rng = np.random.default_rng(3000)
+    mats = [synth.batch_member(rng, i, 3000) for i in range(60)]
+    st = hip.Stream()
+    xs = [DeviceArray.from_numpy(np.random.default_rng(i).uniform(-1, 1, A.n)) for i, A in enumerate(mats)]
+    times = {}
+    for label in ("mlp", "plain_s64"):
+        Ms = []
+        for A in mats:
+            s, v = tuner.pick(model, tuner.matrix_features(A)[0])[0] if label == "mlp" else (64, "balanced")
+            M = AbhsfMatrix(block_size=s, variant=v, stream=st)
+            M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+            Ms.append(M)
+        ys = [DeviceArray.zeros(A.m) for A in mats]
+        B = AbhsfBatch(Ms, st)
+        B.set_vectors(xs, ys)
+        B.spmv_async(1.0, 0.0, graph=True)
+        B.synchronize()
+        for i in (0, 17, 59):
+            A = mats[i]
+            ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, xs[i].numpy())
+            assert np.max(np.abs(ys[i].numpy() - ref)) <= 1e-12 * max(np.max(np.abs(ref)), 1.0)
+        best = []
+        for _ in range(3):
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(st)
+            for _ in range(5):
+                B.spmv_async(1.0, 1.0, graph=True)
+            e1.record(st)
+            e1.synchronize()
+            best.append(e0.elapsed_ms(e1) / 5)
+        times[label] = min(best)
+        B.release()
+    print(f"batched launch: mlp picks {times['mlp']:.3f} ms, plain s=64 {times['plain_s64']:.3f} ms "
+          f"({times['plain_s64'] / times['mlp']:.2f}x)")
+    assert times["mlp"] < times["plain_s64"]
+
+
 def test_blockstats_all_64_pairs_and_tables(gpu):
     """The GPU block-statistics pipeline (abhsf_amd/blockstats.py) reproduces
     the oracle's 64 histograms; the tables built on them are the ones checked
