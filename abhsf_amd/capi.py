@@ -17,6 +17,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_UNSUPPORTED = range(6)
 POLICY_PREPROC, POLICY_MSMF = 0, 1
 F64, F32 = 0, 1
 VARIANT_AUTO, VARIANT_STRIP, VARIANT_BALANCED, VARIANT_PANEL = 0, 1, 2, 3
+GENERAL, SYMMETRIC, SKEW_SYMMETRIC = 0, 1, 2
 SCHEMES = ("coo", "csr", "bitmap", "dense")
 
 # every symbol include/abhsf_c.h declares
@@ -25,7 +26,7 @@ EXPORTED = (
     "abhsf_spmm", "abhsf_export", "abhsf_import", "abhsf_block_hist", "abhsf_mlp_num_params", "abhsf_mlp_forward",
     "abhsf_mlp_backward", "abhsf_mlp_train", "abhsf_batch_create", "abhsf_batch_set_vectors",
     "abhsf_batch_spmv", "abhsf_batch_synchronize", "abhsf_batch_profile", "abhsf_batch_destroy", "abhsf_synchronize",
-    "abhsf_destroy", "abhsf_last_error", "abhsf_version",
+    "abhsf_destroy", "abhsf_last_error", "abhsf_version", "abhsf_set_symmetry",
 )
 
 
@@ -99,6 +100,7 @@ def lib():
         L.abhsf_batch_spmv.argtypes = [vp, d, d, i32]
         L.abhsf_batch_synchronize.argtypes = [vp]
         L.abhsf_batch_profile.argtypes = [vp, d, d, vp]
+        L.abhsf_set_symmetry.argtypes = [vp, i32]
         L.abhsf_batch_destroy.argtypes = [vp]
         L.abhsf_synchronize.argtypes = [vp]
         L.abhsf_destroy.argtypes = [vp]

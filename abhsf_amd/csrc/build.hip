@@ -54,6 +54,32 @@ __global__ void k_check_rowptr(const int64_t *__restrict__ rp, int64_t m, int64_
     if (r < m && rp[r + 1] < rp[r]) atomicOr(err, 4u);
 }
 
+// Triangle flags of a CSR matrix (columns sorted within rows): bit 0 some
+// entry below the diagonal, bit 1 some entry above, bit 2 some diagonal entry.
+__global__ void k_triangle(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int64_t m,
+                           uint32_t *flags)
+{
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    uint32_t f = 0;
+    if (r < m && rp[r + 1] > rp[r]) {
+        int64_t lo = rp[r], hi = rp[r + 1] - 1;
+        if (col[lo] < r) f |= 1u;
+        if (col[hi] > r) f |= 2u;
+        while (lo < hi) {  // first column >= r
+            const int64_t mid = (lo + hi) >> 1;
+            if (col[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        if (col[lo] == r) f |= 4u;
+    }
+    // one atomic per wave and flag value
+    const uint64_t any = __ballot(f != 0);
+    if (any) {
+        uint32_t all = f;
+        for (int o = 32; o >= 1; o >>= 1) all |= (uint32_t)__shfl_xor((int)all, o, 64);
+        if ((threadIdx.x & 63) == 0) atomicOr(flags, all);
+    }
+}
+
 __global__ void k_rows_keys(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int64_t m, int64_t n,
                             uint64_t nnz, int k, int jbits, uint32_t *__restrict__ rows,
                             uint64_t *__restrict__ key, uint32_t *__restrict__ perm, uint32_t *err)
@@ -283,6 +309,7 @@ SpmvLaunch Storage::launch() const
     SpmvLaunch L;
     L.bcol_bytes = bcol_bytes;
     L.val_bytes = val_bytes;
+    L.coo_single = nb[COO] > 0 && 4 * ne[COO] < 5 * nb[COO];  // mean entries per COO block below 1.25
     L.bal.light = bal_light.get<uint32_t>();
     L.bal.pack = bal_pack.get<uint32_t>();
     L.bal.units = bal_units.get<uint32_t>();
@@ -407,7 +434,7 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels)
                 const int64_t wz = work(x, b);
                 const int p = panels > 1 ? panel_of[bc[x][b]] : 0;
                 if (b > ub && (p != up || acc + wz > unit)) {
-                    units.insert(units.end(), {(uint32_t)x, ub, b, (uint32_t)ue});
+                    units.insert(units.end(), {(uint32_t)x | ((uint32_t)I << 2), ub, b, (uint32_t)ue});
                     unit_panel.push_back((uint8_t)up);
                     ub = b;
                     ue = e;
@@ -418,7 +445,7 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels)
                 e += (uint64_t)wz;
             }
             if (ub < b1) {
-                units.insert(units.end(), {(uint32_t)x, ub, b1, (uint32_t)ue});
+                units.insert(units.end(), {(uint32_t)x | ((uint32_t)I << 2), ub, b1, (uint32_t)ue});
                 unit_panel.push_back((uint8_t)up);
             }
         }
@@ -509,6 +536,15 @@ uint64_t Storage::stored_bytes() const
                  bal_order.bytes();
     for (int x = 0; x < 4; x++) b += sptr[x].bytes() + eptr[x].bytes() + bcol[x].bytes() + z1[x].bytes();
     return b;
+}
+
+uint32_t csr_triangle_flags(const int64_t *rowptr, const int32_t *col, int64_t m, hipStream_t st)
+{
+    DevBuf f(sizeof(uint32_t));
+    ABHSF_HIP(hipMemsetAsync(f.get(), 0, 4, st));
+    k_triangle<<<grid_for((uint64_t)m), kT, 0, st>>>(rowptr, col, m, f.get<uint32_t>());
+    ABHSF_HIP(hipGetLastError());
+    return read_dev(f.get<uint32_t>(), st);
 }
 
 void build_from_device_csr(Storage &S, const int64_t *rowptr, const int32_t *col, const double *val, hipStream_t st)

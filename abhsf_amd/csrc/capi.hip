@@ -28,6 +28,8 @@ struct abhsf_handle {
     abhsf_options opts;
     hipStream_t stream;
     abhsf::Storage S;
+    int32_t symmetry = ABHSF_GENERAL;  // applies to the next build
+    int32_t built_symmetry = ABHSF_GENERAL;
 };
 
 struct abhsf_batch {
@@ -36,7 +38,7 @@ struct abhsf_batch {
     std::vector<abhsf::BatchEntry> host;
     abhsf::DevBuf table, heavy, item_mat;
     int64_t total_items = 0, nheavy = 0;
-    int grid = 1;
+    int64_t code_begin[abhsf::kBatchCodes] = {}, code_end[abhsf::kBatchCodes] = {};
     bool vectors_set = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -97,6 +99,32 @@ void finish_build(abhsf_handle *h)
                            h->opts.variant == ABHSF_VARIANT_PANEL ? abhsf::kXcdCount : 1);
 }
 
+void check_symmetric_shape(const abhsf_handle *h, int64_t m, int64_t n)
+{
+    if (h->symmetry == ABHSF_GENERAL) return;
+    ABHSF_REQUIRE(m == n, ABHSF_ERR_INVALID, "symmetric storage needs a square matrix");
+    ABHSF_REQUIRE(h->opts.precision == ABHSF_F64, ABHSF_ERR_UNSUPPORTED, "symmetric storage needs FP64 values");
+}
+
+// Symmetric storage: the stored entries must lie in one triangle
+// (matrix_market_reader.h:191-197 rejects both), skew-symmetric has no
+// diagonal; the CSR arrays are valid (the build checked them).
+void check_symmetry(abhsf_handle *h, const int64_t *d_rowptr, const int32_t *d_col)
+{
+    h->built_symmetry = ABHSF_GENERAL;
+    if (h->symmetry == ABHSF_GENERAL) return;
+    const uint32_t f = abhsf::csr_triangle_flags(d_rowptr, d_col, h->S.m, h->stream);
+    if ((f & 3u) == 3u) {
+        h->S.built = false;
+        throw abhsf::Error(ABHSF_ERR_INVALID, "Elements from both L and U parts stored for not-unsymmetric matrix.");
+    }
+    if (h->symmetry == ABHSF_SKEW_SYMMETRIC && (f & 4u)) {
+        h->S.built = false;
+        throw abhsf::Error(ABHSF_ERR_INVALID, "skew-symmetric matrix with a diagonal entry");
+    }
+    h->built_symmetry = h->symmetry;
+}
+
 void check_shape(int64_t m, int64_t n, int64_t nnz)
 {
     ABHSF_REQUIRE(m >= 1 && n >= 1 && m <= INT32_MAX && n <= INT32_MAX, ABHSF_ERR_INVALID,
@@ -148,9 +176,11 @@ abhsf_status abhsf_build_csr(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
         ABHSF_REQUIRE(h && rowptr, ABHSF_ERR_INVALID, "null argument");
         ABHSF_REQUIRE(nnz == 0 || col, ABHSF_ERR_INVALID, "null column array");
         check_shape(m, n, nnz);
+        check_symmetric_shape(h, m, n);
         reset_storage(h, m, n, nnz);
         if (on_device) {
             abhsf::build_from_device_csr(h->S, rowptr, col, val, h->stream);
+            check_symmetry(h, rowptr, col);
             finish_build(h);
             return;
         }
@@ -161,6 +191,7 @@ abhsf_status abhsf_build_csr(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
         if (val && nnz) ABHSF_HIP(hipMemcpyAsync(dv.get(), val, dv.bytes(), hipMemcpyHostToDevice, h->stream));
         abhsf::build_from_device_csr(h->S, drp.get<int64_t>(), dc.get<int32_t>(), val ? dv.get<double>() : nullptr,
                                      h->stream);
+        check_symmetry(h, drp.get<int64_t>(), dc.get<int32_t>());
         finish_build(h);
     });
 }
@@ -172,6 +203,7 @@ abhsf_status abhsf_build_coo(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
         ABHSF_REQUIRE(h, ABHSF_ERR_INVALID, "null handle");
         ABHSF_REQUIRE(nnz == 0 || (rows && cols), ABHSF_ERR_INVALID, "null triplet array");
         check_shape(m, n, nnz);
+        check_symmetric_shape(h, m, n);
         for (int64_t i = 0; i < nnz; i++)
             ABHSF_REQUIRE(rows[i] >= 0 && rows[i] < m && cols[i] >= 0 && cols[i] < n, ABHSF_ERR_INVALID,
                           "triplet index out of range");
@@ -203,6 +235,7 @@ abhsf_status abhsf_build_coo(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
         if (vals && nnz) ABHSF_HIP(hipMemcpyAsync(dv.get(), v.data(), dv.bytes(), hipMemcpyHostToDevice, h->stream));
         abhsf::build_from_device_csr(h->S, drp.get<int64_t>(), dc.get<int32_t>(), vals ? dv.get<double>() : nullptr,
                                      h->stream);
+        check_symmetry(h, drp.get<int64_t>(), dc.get<int32_t>());
         finish_build(h);
     });
 }
@@ -212,6 +245,8 @@ abhsf_status abhsf_import(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz, in
 {
     return guarded([&] {
         ABHSF_REQUIRE(h && names && data && bytes && count > 0, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(h->symmetry == ABHSF_GENERAL, ABHSF_ERR_UNSUPPORTED,
+                      "a serialised blob holds general storage (build symmetric storage from CSR)");
         check_shape(m, n, nnz);
         reset_storage(h, m, n, nnz);
         abhsf::import_storage(h->S, count, names, data, bytes, h->stream);
@@ -246,6 +281,20 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
         ABHSF_REQUIRE(nrhs >= 1 && nrhs <= 64, ABHSF_ERR_UNSUPPORTED, "nrhs must be in [1, 64]");
         ABHSF_REQUIRE(ldx >= nrhs && ldy >= nrhs, ABHSF_ERR_INVALID, "leading dimension smaller than nrhs");
         abhsf::spmm(h->S.launch(), h->S.view(), nrhs, alpha, X, ldx, beta, Y, ldy, h->stream);
+        if (h->built_symmetry != ABHSF_GENERAL && alpha != 0.0)  // the mirrored triangle
+            abhsf::spmv_transposed(h->S.launch(), h->S.view(), nrhs,
+                                   h->built_symmetry == ABHSF_SKEW_SYMMETRIC ? -alpha : alpha, X, ldx, Y, ldy,
+                                   h->stream);
+    });
+}
+
+abhsf_status abhsf_set_symmetry(abhsf_handle *h, int32_t symmetry)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(h, ABHSF_ERR_INVALID, "null handle");
+        ABHSF_REQUIRE(symmetry == ABHSF_GENERAL || symmetry == ABHSF_SYMMETRIC || symmetry == ABHSF_SKEW_SYMMETRIC,
+                      ABHSF_ERR_INVALID, "unknown symmetry");
+        h->symmetry = symmetry;
     });
 }
 
@@ -345,35 +394,43 @@ abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *
             b->own_stream = true;
         }
         std::vector<uint32_t> hv;
-        int64_t item = 0;
         for (int32_t i = 0; i < count; i++) {
             const abhsf_handle *h = mats[i];
             ABHSF_REQUIRE(h && h->S.built, ABHSF_ERR_STATE, "batch member not built");
             ABHSF_REQUIRE(h->S.val_bytes == 8, ABHSF_ERR_UNSUPPORTED, "batched launch needs FP64 storage");
+            ABHSF_REQUIRE(h->built_symmetry == ABHSF_GENERAL, ABHSF_ERR_UNSUPPORTED,
+                          "batched launch takes general (both-triangle) storage");
             abhsf::BatchEntry e{};
             e.A = h->S.view();
             e.B = h->S.launch().bal;
             if (!e.B.light && e.B.n_units == 0) e.B.n_light = h->S.M;
-            e.item0 = item;
             e.code = abhsf::batch_code(h->S.bcol_bytes, h->S.s);
-            item += e.B.n_light + e.B.n_units;
             for (int64_t q = 0; q < e.B.n_heavy; q++) hv.insert(hv.end(), {(uint32_t)i, (uint32_t)q});
             b->host.push_back(e);
+        }
+        // number the items grouped by index-type combination (one kernel per
+        // combination, each with the register budget of its own variant)
+        int64_t item = 0;
+        for (int c = 0; c < abhsf::kBatchCodes; c++) {
+            b->code_begin[c] = item;
+            for (auto &e : b->host)
+                if (e.code == c) {
+                    e.item0 = item;
+                    item += e.B.n_light + e.B.n_units;
+                }
+            b->code_end[c] = item;
         }
         b->total_items = item;
         b->nheavy = (int64_t)hv.size() / 2;
         ABHSF_REQUIRE(item < (int64_t)UINT32_MAX, ABHSF_ERR_UNSUPPORTED, "too many work items in one batch");
         std::vector<uint32_t> im((size_t)item);
         for (size_t i = 0; i < b->host.size(); i++) {
-            const int64_t i0 = b->host[i].item0;
-            const int64_t i1 = (i + 1 < b->host.size()) ? b->host[i + 1].item0 : item;
-            std::fill(im.begin() + i0, im.begin() + i1, (uint32_t)i);
+            const auto &e = b->host[i];
+            std::fill(im.begin() + e.item0, im.begin() + e.item0 + e.B.n_light + e.B.n_units, (uint32_t)i);
         }
         b->item_mat.alloc(im.size() * 4);
         if (!im.empty())
             ABHSF_HIP(hipMemcpyAsync(b->item_mat.get(), im.data(), im.size() * 4, hipMemcpyHostToDevice, b->stream));
-        const int64_t want = (item + 3) / 4;
-        b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 8192));
         b->table.alloc(b->host.size() * sizeof(abhsf::BatchEntry));
         b->heavy.alloc(hv.size() * 4);
         if (!hv.empty())
@@ -404,8 +461,8 @@ abhsf_status abhsf_batch_spmv(abhsf_batch *b, double alpha, double beta, int32_t
         ABHSF_REQUIRE(b, ABHSF_ERR_INVALID, "null batch");
         ABHSF_REQUIRE(b->vectors_set, ABHSF_ERR_STATE, "abhsf_batch_set_vectors not called");
         auto launch = [&] {
-            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
-                              b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->grid, b->stream);
+            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->code_begin,
+                              b->code_end, b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream);
         };
         if (!use_graph) {
             launch();
@@ -450,8 +507,8 @@ abhsf_status abhsf_batch_profile(abhsf_batch *b, double alpha, double beta, uint
         const size_t nm = b->host.size();
         abhsf::DevBuf cyc(nm * sizeof(unsigned long long));
         ABHSF_HIP(hipMemsetAsync(cyc.get(), 0, cyc.bytes(), b->stream));
-        abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
-                          b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->grid, b->stream,
+        abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->code_begin,
+                          b->code_end, b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream,
                           cyc.get<unsigned long long>());
         ABHSF_HIP(hipMemcpyAsync(cycles, cyc.get(), cyc.bytes(), hipMemcpyDeviceToHost, b->stream));
         ABHSF_HIP(hipStreamSynchronize(b->stream));

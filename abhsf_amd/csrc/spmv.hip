@@ -420,7 +420,7 @@ __device__ inline int64_t xcd_block_id()
 
 // One work item of one matrix (see Balance): the wave accumulates in acc_buf
 // (s doubles, 2s when compensated) and writes y rows or a unit's partials.
-template <typename VT, typename BT, typename IT, typename RT, bool COMP>
+template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS = 2, int CH = 4>
 __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item, const double *__restrict__ x,
                                  double *__restrict__ y, double alpha, double beta, double *acc_buf,
                                  uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, int lane)
@@ -451,7 +451,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
                 }
                 wave_sync();
             }
-            coo_r1<VT, BT, IT, COMP, 2, 4>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc,
+            coo_r1<VT, BT, IT, COMP, ZS, CH>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc,
                                         s_st, lane);
             csr_rows_r1<VT, BT, RT, COMP>(A, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc, s_st, s_bc,
                                           lane);
@@ -474,7 +474,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         const uint32_t *U = B.units + u * 4;
         const uint32_t sc = U[0] & 3u, b0 = U[1], b1 = U[2], e0 = U[3];
         if (sc == COO)
-            coo_r1<VT, BT, IT, COMP, 2, 4>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, lane);
+            coo_r1<VT, BT, IT, COMP, ZS, CH>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, lane);
         else if (sc == CSR)
             csr_rows_r1<VT, BT, RT, COMP>(A, b0, b1, e0, x, acc, s_st, s_bc, lane);
         else
@@ -494,7 +494,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
 // light) -- whose rows are written to y; items n_light + u are
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
-template <typename VT, typename BT, typename IT, typename RT, bool COMP>
+template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS, int CH>
 __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__restrict__ x, double *__restrict__ y,
                                                     double alpha, double beta, Balance B)
 {
@@ -513,19 +513,159 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
         item = xcd_block_id() * kWaves + wave;
     }
     if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
-    spmv_item<VT, BT, IT, RT, COMP>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave], s_bc[wave], s_st[wave],
-                                    lane);
+    spmv_item<VT, BT, IT, RT, COMP, ZS, CH>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave], s_bc[wave],
+                                            s_st[wave], lane);
 }
 
-// Many matrices in one launch: wave w takes global items w, w + W, ... (W =
-// all waves of the grid); item_mat[g] names the matrix of global item g (one
-// load instead of a search over the table); the block-index types are
-// dispatched per matrix.  FP64 values.  TIMED: each wave also adds the
-// shader-clock cycles of its item to cyc[matrix] (the per-matrix cost inside
-// a full launch).
-template <bool TIMED>
+// Many matrices in one launch: the waves take the global items [g0, g1) of
+// one index-type combination, wave w items g0 + w, g0 + w + W, ... (W = all
+// waves of the grid); item_mat[g] names the matrix of item g.  FP64 values.
+// TIMED: each wave also adds the shader-clock cycles of its item to
+// cyc[matrix] (the per-matrix cost inside a full launch).
+// ===========================================================================
+// Symmetric storage (one triangle T stored): the main pass computes
+// y = alpha*T*x + beta*y, this pass adds scale * T_off^T * x (scale = alpha,
+// or -alpha for skew-symmetric; T_off = T without its diagonal): for every
+// stored entry (i, j), i != j, y[j] += scale * a_ij * x[i], by global FP64
+// atomics (entries of a strip scatter over the column range).  FP64 values.
+// ===========================================================================
+template <typename BT, typename IT, typename RT>
+__device__ inline void trans_group(const DevMat &A, int sc, uint32_t b0, uint32_t b1, uint64_t e, int64_t I,
+                                   const double *__restrict__ x, int64_t ldx, double *y, int64_t ldy, double scale,
+                                   uint32_t *s_end, uint32_t *s_bc, int lane)
+{
+    if (b0 >= b1) return;
+    const int s = A.s, k = A.k;
+    const uint32_t smask = (uint32_t)s - 1, ss = (uint32_t)s * (uint32_t)s;
+    const double *__restrict__ val = static_cast<const double *>(A.val[sc]);
+    const int64_t r0 = I * s;
+    auto put = [&](uint32_t lr, int64_t gc, double v) {
+        const int64_t gr = r0 + lr;
+        if (gr != gc) atomicAdd(&y[gc * ldy], scale * v * x[gr * ldx]);
+    };
+    if (sc == COO || sc == CSR) {
+        // element-parallel over chunks of 64 blocks; an entry's block by binary
+        // search over the chunk's prefix sums, its CSR row over the block's row
+        // pointers
+#pragma unroll 1
+        for (uint32_t cb = b0; cb < b1; cb += kWave) {
+            const uint32_t jb = cb + lane;
+            const bool ok = jb < b1;
+            const uint32_t z = ok ? (uint32_t)A.z1[sc][jb] + 1u : 0u;
+            const uint32_t incl = wave_incl_scan_dpp(z);
+            const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+            s_end[lane] = incl;
+            s_bc[lane] = ok ? (uint32_t)ld<BT>(A.bcol[sc], jb) : 0u;
+            wave_sync();
+            const int nbk = (int)min<uint32_t>(kWave, b1 - cb);
+#pragma unroll 1
+            for (uint32_t q = 0; q < T; q += kWave) {
+                const uint32_t t = q + lane;
+                if (t < T) {
+                    int lo = 0, hi = nbk - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_end[mid] > t) hi = mid; else lo = mid + 1;
+                    }
+                    uint32_t lr, lc;
+                    if (sc == COO) {
+                        const uint32_t id = ld<IT>(A.coo_idx, e + t);
+                        lr = id >> k;
+                        lc = id & smask;
+                    } else {
+                        const uint32_t qb = t - (lo ? s_end[lo - 1] : 0u);
+                        const RT *rp = static_cast<const RT *>(A.csr_rowptr) + (uint64_t)(cb + lo) * (s + 1);
+                        int rl = 0, rh = s - 1;
+                        while (rl < rh) {
+                            const int mid = (rl + rh + 1) >> 1;
+                            if ((uint32_t)rp[mid] <= qb) rl = mid; else rh = mid - 1;
+                        }
+                        lr = (uint32_t)rl;
+                        lc = A.csr_col[e + t];
+                    }
+                    put(lr, (int64_t)s_bc[lo] * s + lc, val[e + t]);
+                }
+            }
+            e += T;
+            wave_sync();
+        }
+        return;
+    }
+    // bitmap / dense: one block at a time, lanes over its positions
+#pragma unroll 1
+    for (uint32_t b = b0; b < b1; b++) {
+        const int64_t c0 = (int64_t)ld<BT>(A.bcol[sc], b) * s;
+        if (sc == DENSE) {
+            const uint64_t db = (uint64_t)b * ss;
+            for (uint32_t pos = (uint32_t)lane; pos < ss; pos += kWave) {
+                const double v = val[db + pos];
+                if (v != 0.0) put(pos >> k, c0 + (pos & smask), v);
+            }
+        } else {
+            const uint64_t *__restrict__ words = A.bm_words + (uint64_t)b * A.W;
+#pragma unroll 1
+            for (int w = 0; w < A.W; w++) {
+                const uint64_t word = words[w];
+                const uint32_t pos = (uint32_t)w * 64u + (uint32_t)lane;
+                if ((word >> lane) & 1ull)
+                    put(pos >> k, c0 + (pos & smask), val[e + __popcll(word & ((1ull << lane) - 1ull))]);
+                e += (uint64_t)__popcll(word);
+            }
+        }
+    }
+}
+
+template <typename BT, typename IT, typename RT>
+__global__ __launch_bounds__(256) void k_spmv_trans(DevMat A, Balance B, const double *__restrict__ x, int64_t ldx,
+                                                    double *y, int64_t ldy, double scale)
+{
+    constexpr int kWaves = 4;
+    __shared__ uint32_t s_end[kWaves][kWave];
+    __shared__ uint32_t s_bc[kWaves][kWave];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * kWaves + wave;
+    if (item >= B.n_light + B.n_units) return;
+    uint32_t *se = s_end[wave], *sb = s_bc[wave];
+    if (item < B.n_light) {
+        uint32_t p0 = (uint32_t)item, p1 = (uint32_t)item + 1u;
+        if (B.pack) {
+            p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item]);
+            p1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item + 1]);
+        }
+#pragma unroll 1
+        for (uint32_t p = p0; p < p1; p++) {
+            const int64_t I = B.light ? (int64_t)__builtin_amdgcn_readfirstlane((int)B.light[p]) : (int64_t)p;
+#pragma unroll 1
+            for (int sc = COO; sc <= DENSE; sc++)
+                trans_group<BT, IT, RT>(A, sc, A.sptr[sc][I], A.sptr[sc][I + 1],
+                                        sc == DENSE ? 0 : A.eptr[sc][I], I, x, ldx, y, ldy, scale, se, sb, lane);
+        }
+    } else {
+        const uint32_t *U = B.units + (item - B.n_light) * 4;
+        const uint32_t u0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)U[0]);
+        trans_group<BT, IT, RT>(A, (int)(u0 & 3u), U[1], U[2], U[3], (int64_t)(u0 >> 2), x, ldx, y, ldy, scale, se,
+                                sb, lane);
+    }
+}
+
+// Copy of a wave-uniform table entry, read through the constant address
+// space (the table does not change during a launch) so that the compiler
+// uses scalar loads and keeps the fields in SGPRs.
+template <typename T>
+__device__ inline T load_uniform(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "word-sized struct expected");
+    T out;
+    const auto *src = (const __attribute__((address_space(4))) uint32_t *)p;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 4; i++) dst[i] = src[i];
+    return out;
+}
+
+template <typename BT, typename IT, typename RT, bool TIMED>
 __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict__ E,
-                                                    const uint32_t *__restrict__ item_mat, int64_t total,
+                                                    const uint32_t *__restrict__ item_mat, int64_t g0, int64_t g1,
                                                     double alpha, double beta, unsigned long long *cyc)
 {
     constexpr int kWaves = 4;
@@ -533,28 +673,22 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
     __shared__ uint32_t s_end[kWaves][kWave];
     __shared__ uint32_t s_bc[kWaves][kWave];
     __shared__ uint32_t s_st[kWaves][kWave];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * kWaves;
 #pragma unroll 1
-    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < total; g += nw) {
-        const int lo = __builtin_amdgcn_readfirstlane((int)item_mat[g]);
-        const BatchEntry &e = E[lo];
-        const int64_t it = g - e.item0;
-        double *acc = s_acc[wave];
-        uint32_t *se = s_end[wave], *sb = s_bc[wave], *ss = s_st[wave];
+    for (int64_t g = g0 + (int64_t)blockIdx.x * kWaves + wave; g < g1; g += nw) {
+        const int mat = __builtin_amdgcn_readfirstlane((int)item_mat[g]);
+        // the table is read-only during the launch: read the entry through the
+        // constant address space so its fields are scalar loads into SGPRs
+        // (as the single-matrix kernel gets them from its arguments)
+        const BatchEntry e = load_uniform(E + mat);
         long long t0 = 0;
         if constexpr (TIMED) t0 = clock64();
-        switch (e.code) {
-        case 0: spmv_item<double, uint16_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
-        case 1: spmv_item<double, uint16_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
-        case 2: spmv_item<double, uint16_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
-        case 3: spmv_item<double, uint32_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
-        case 4: spmv_item<double, uint32_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
-        default: spmv_item<double, uint32_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, acc, se, sb, ss, lane); break;
-        }
+        spmv_item<double, BT, IT, RT, false>(e.A, e.B, g - e.item0, e.x, e.y, alpha, beta, s_acc[wave],
+                                             s_end[wave], s_bc[wave], s_st[wave], lane);
         if constexpr (TIMED) {
             const long long t1 = clock64();
-            if (lane == 0) atomicAdd(&cyc[lo], (unsigned long long)(t1 - t0));
+            if (lane == 0) atomicAdd(&cyc[mat], (unsigned long long)(t1 - t0));
         }
     }
 }
@@ -651,6 +785,14 @@ __global__ __launch_bounds__(kFixThreads) void k_spmv_fixup(int s, int64_t m, do
     fixup_strip<COMP>(s, m, B.heavy[h * 3], B.part, B.heavy[h * 3 + 1], B.heavy[h * 3 + 2], y, alpha, beta, sh);
 }
 
+// LDS bytes per wave of the SpMM kernel: the s x R accumulator (twice when
+// compensated), the decoded COO window (value, row, column) and the chunk's
+// prefix sums and block columns.
+__host__ __device__ inline size_t spmm_wave_lds(int s, int R, bool comp)
+{
+    return (size_t)s * R * sizeof(double) * (comp ? 2 : 1) + kWave * (sizeof(double) + 4 * sizeof(uint32_t));
+}
+
 // ===========================================================================
 // Several right-hand sides (SpMM), R lanes per entry.
 // ===========================================================================
@@ -701,13 +843,16 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
     const int64_t I = (int64_t)blockIdx.x * wpb + wave;
     if (I >= nstrip_total) return;  // whole wave leaves; no workgroup barriers below
 
-    const size_t per_wave = (size_t)s * R * sizeof(double) * (COMP ? 2 : 1) + 2 * kWave * sizeof(uint32_t);
+    const size_t per_wave = spmm_wave_lds(s, R, COMP);
     unsigned char *base = smem + per_wave * wave;
     Acc acc;
     acc.hi = reinterpret_cast<double *>(base);
     acc.lo = COMP ? acc.hi + (size_t)s * R : nullptr;
-    uint32_t *s_end = reinterpret_cast<uint32_t *>(base + (size_t)s * R * sizeof(double) * (COMP ? 2 : 1));
+    double *d_val = reinterpret_cast<double *>(base + (size_t)s * R * sizeof(double) * (COMP ? 2 : 1));
+    uint32_t *s_end = reinterpret_cast<uint32_t *>(d_val + kWave);
     uint32_t *s_bc = s_end + kWave;
+    uint32_t *d_row = s_bc + kWave;
+    uint32_t *d_col = d_row + kWave;
 
     for (int i = lane; i < s * R; i += kWave) {
         acc.hi[i] = 0.0;
@@ -717,59 +862,92 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
 
     const uint32_t smask = (uint32_t)s - 1;
 
-    // ---- COO and CSR groups: element-parallel over 64-block chunks ----------
-#pragma unroll 1
-    for (int sc = COO; sc <= CSR; sc++) {
-        const uint32_t b0 = A.sptr[sc][I], b1 = A.sptr[sc][I + 1];
-        uint32_t e = A.eptr[sc][I];
-        const VT *val = static_cast<const VT *>(A.val[sc]);
+    // ---- COO group: windows of 64 entries are decoded once (one entry per
+    // lane: block by binary search over the chunk's prefix sums, local row and
+    // global column, value) into LDS, then E = 64/R entries per step times R
+    // right-hand sides read the decoded window -------------------------------
+    {
+        const uint32_t b0 = A.sptr[COO][I], b1 = A.sptr[COO][I + 1];
+        uint32_t e = A.eptr[COO][I];
+        const VT *val = static_cast<const VT *>(A.val[COO]);
 #pragma unroll 1
         for (uint32_t cb = b0; cb < b1; cb += kWave) {
             const uint32_t jb = cb + lane;
             const bool ok = jb < b1;
-            const uint32_t z = ok ? (uint32_t)A.z1[sc][jb] + 1u : 0u;
-            const uint32_t incl = wave_incl_scan_u32(z);
-            const uint32_t T = __shfl(incl, kWave - 1, kWave);
+            const uint32_t z = ok ? (uint32_t)A.z1[COO][jb] + 1u : 0u;
+            const uint32_t incl = wave_incl_scan_dpp(z);
+            const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
             s_end[lane] = incl;
-            s_bc[lane] = ok ? (uint32_t)ld<BT>(A.bcol[sc], jb) : 0u;
+            s_bc[lane] = ok ? (uint32_t)ld<BT>(A.bcol[COO], jb) : 0u;
             wave_sync();
             const int nbk = (int)min<uint32_t>(kWave, b1 - cb);
 #pragma unroll 1
-            for (uint32_t q = 0; q < T; q += E) {
-                const uint32_t t = q + slot;
+            for (uint32_t q = 0; q < T; q += kWave) {
+                const uint32_t t = q + lane;
                 if (t < T) {
                     int lo = 0, hi = nbk - 1;  // first chunk block with end > t
                     while (lo < hi) {
-                        int mid = (lo + hi) >> 1;
+                        const int mid = (lo + hi) >> 1;
                         if (s_end[mid] > t) hi = mid; else lo = mid + 1;
                     }
-                    uint32_t lr, lc;
-                    if (sc == COO) {
-                        const uint32_t id = ld<IT>(A.coo_idx, e + t);
-                        lr = id >> k;
-                        lc = id & smask;
-                    } else {
-                        const uint32_t qb = t - (lo ? s_end[lo - 1] : 0u);  // entry within block
-                        const RT *rp = static_cast<const RT *>(A.csr_rowptr) + (uint64_t)(cb + lo) * (s + 1);
-                        int rl = 0, rh = s - 1;  // last row with rp[row] <= qb
-                        while (rl < rh) {
-                            int mid = (rl + rh + 1) >> 1;
-                            if ((uint32_t)rp[mid] <= qb) rl = mid; else rh = mid - 1;
-                        }
-                        lr = (uint32_t)rl;
-                        lc = A.csr_col[e + t];
-                    }
-                    if (jok) {
-                        const double v = (double)val[e + t];
-                        const double xv = X[((int64_t)s_bc[lo] * s + lc) * ldx + j];
+                    const uint32_t id = ld<IT>(A.coo_idx, e + t);
+                    d_row[lane] = id >> k;
+                    d_col[lane] = s_bc[lo] * (uint32_t)s + (id & smask);
+                    d_val[lane] = (double)val[e + t];
+                }
+                wave_sync();
+                const uint32_t nq = min<uint32_t>(kWave, T - q);
+#pragma unroll 1
+                for (uint32_t g = 0; g < nq; g += E) {
+                    const uint32_t ent = g + slot;
+                    if (ent < nq && jok) {
+                        const double xv = X[(int64_t)d_col[ent] * ldx + j];
                         double pe;
-                        const double p = prod<COMP>(v, xv, pe);
-                        acc_atomic<COMP>(acc, lr * R + j, p, pe);
+                        const double p = prod<COMP>(d_val[ent], xv, pe);
+                        acc_atomic<COMP>(acc, d_row[ent] * R + j, p, pe);
+                    }
+                }
+                wave_sync();
+            }
+            e += T;
+        }
+    }
+
+    // ---- CSR group: one block at a time, E rows per step, R lanes per row
+    // (one per right-hand side); a lane walks its row's entries and adds the
+    // row sum to the accumulator (rows of one step are distinct: no atomics)
+    {
+        const uint32_t b0 = A.sptr[CSR][I], b1 = A.sptr[CSR][I + 1];
+        uint64_t e = A.eptr[CSR][I];
+        const VT *val = static_cast<const VT *>(A.val[CSR]);
+        const RT *rowptr = static_cast<const RT *>(A.csr_rowptr);
+#pragma unroll 1
+        for (uint32_t b = b0; b < b1; b++) {
+            const int64_t c0 = (int64_t)ld<BT>(A.bcol[CSR], b) * s;
+            const RT *rp = rowptr + (uint64_t)b * (uint64_t)(s + 1);
+#pragma unroll 1
+            for (int r0 = 0; r0 < s; r0 += E) {
+                const int r = r0 + slot;
+                if (r < s && jok) {
+                    const uint32_t q0 = rp[r], q1 = rp[r + 1];
+                    double hi = 0.0, lo = 0.0;
+                    for (uint32_t qq = q0; qq < q1; qq++)
+                        acc_add<COMP>(hi, lo, (double)val[e + qq], X[(c0 + A.csr_col[e + qq]) * ldx + j]);
+                    if (q1 > q0) {
+                        const uint32_t ai = (uint32_t)r * R + j;
+                        if constexpr (COMP) {
+                            double sum, err;
+                            two_sum(acc.hi[ai], hi, sum, err);
+                            acc.hi[ai] = sum;
+                            acc.lo[ai] += err + lo;
+                        } else {
+                            acc.hi[ai] += hi;
+                        }
                     }
                 }
             }
-            e += T;
             wave_sync();
+            e += (uint64_t)A.z1[CSR][b] + 1u;
         }
     }
 
@@ -794,6 +972,39 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
             if constexpr (R >= 16 && !COMP) {
                 if (sc == DENSE && s >= 16) {
                     dense_block_mfma<VT, R>(val + vbase, s, c0, A.n, X, ldx, nrhs, acc.hi, lane);
+                    continue;
+                }
+            }
+            if constexpr (R >= 2) {
+                if (sc == BITMAP) {
+                    // decode each word's set bits (in value order) into the
+                    // window, then E entries x R right-hand sides per step:
+                    // no step is spent on empty positions
+#pragma unroll 1
+                    for (int w = 0; w < W; w++) {
+                        const uint64_t word = words[w];
+                        const uint32_t nq = (uint32_t)__popcll(word);
+                        if ((word >> lane) & 1ull) {
+                            const uint32_t ix = (uint32_t)__popcll(word & ((1ull << lane) - 1ull));
+                            const uint32_t pos = (uint32_t)w * 64u + (uint32_t)lane;
+                            d_row[ix] = pos >> k;
+                            d_col[ix] = (uint32_t)(c0 + (pos & smask));
+                            d_val[ix] = (double)val[vbase + ix];
+                        }
+                        wave_sync();
+#pragma unroll 1
+                        for (uint32_t g = 0; g < nq; g += E) {
+                            const uint32_t ent = g + slot;
+                            if (ent < nq && jok) {
+                                const double xv = X[(int64_t)d_col[ent] * ldx + j];
+                                double pe;
+                                const double p = prod<COMP>(d_val[ent], xv, pe);
+                                acc_atomic<COMP>(acc, d_row[ent] * R + j, p, pe);
+                            }
+                        }
+                        wave_sync();
+                        vbase += nq;
+                    }
                     continue;
                 }
             }
@@ -861,7 +1072,7 @@ template <typename VT, typename BT, typename IT, typename RT, int R, bool COMP>
 void launch_t(const DevMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy, int nrhs, double alpha,
               double beta, hipStream_t st)
 {
-    const size_t per_wave = (size_t)A.s * R * sizeof(double) * (COMP ? 2 : 1) + 2 * kWave * sizeof(uint32_t);
+    const size_t per_wave = spmm_wave_lds(A.s, R, COMP);
     int wpb = (int)(65536 / per_wave);
     if (wpb > 4) wpb = 4;
     ABHSF_REQUIRE(wpb >= 1, ABHSF_ERR_UNSUPPORTED, "output tile does not fit in LDS");
@@ -874,8 +1085,8 @@ void launch_t(const DevMat &A, const double *X, int64_t ldx, double *Y, int64_t 
 }
 
 template <typename VT, typename BT, typename IT, typename RT, bool COMP>
-void launch_spmv(const DevMat &A, const Balance &B, const double *x, double *y, double alpha, double beta,
-                 hipStream_t st)
+void launch_spmv(const DevMat &A, const Balance &B, bool coo_single, const double *x, double *y, double alpha,
+                 double beta, hipStream_t st)
 {
     if (A.M == 0) return;
     Balance b = B;
@@ -883,8 +1094,15 @@ void launch_spmv(const DevMat &A, const Balance &B, const double *x, double *y, 
     const int64_t nitems = b.n_light + b.n_units;
     const int64_t nblk = b.order ? b.n_slots / 4 : (nitems + 3) / 4;
     ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many strips");
-    hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP>), dim3((unsigned)nblk), dim3(256), 0, st, A, x, y, alpha,
-                       beta, b);
+    // COO blocks of (almost) one entry: phase A takes one entry from each of
+    // 8 x 64 blocks per step (more gathers in flight, no unused entry slot);
+    // otherwise two entries from each of 4 x 64 blocks
+    if (coo_single)
+        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 8>), dim3((unsigned)nblk), dim3(256), 0, st, A, x,
+                           y, alpha, beta, b);
+    else
+        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 2, 4>), dim3((unsigned)nblk), dim3(256), 0, st, A, x,
+                           y, alpha, beta, b);
     ABHSF_HIP(hipGetLastError());
     if (b.n_heavy) {
         hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)b.n_heavy), dim3(kFixThreads), 0, st, A.s,
@@ -894,12 +1112,13 @@ void launch_spmv(const DevMat &A, const Balance &B, const double *x, double *y, 
 }
 
 template <typename VT, typename BT, typename IT, typename RT, bool COMP>
-void launch_r(int R, const DevMat &A, const Balance &Bal, const double *X, int64_t ldx, double *Y, int64_t ldy,
-              int nrhs, double alpha, double beta, hipStream_t st)
+void launch_r(int R, const DevMat &A, const Balance &Bal, bool coo_single, const double *X, int64_t ldx, double *Y,
+              int64_t ldy, int nrhs, double alpha, double beta, hipStream_t st)
 {
     switch (R) {
     case 1:
-        if (nrhs == 1 && ldx == 1 && ldy == 1) launch_spmv<VT, BT, IT, RT, COMP>(A, Bal, X, Y, alpha, beta, st);
+        if (nrhs == 1 && ldx == 1 && ldy == 1)
+            launch_spmv<VT, BT, IT, RT, COMP>(A, Bal, coo_single, X, Y, alpha, beta, st);
         else launch_t<VT, BT, IT, RT, 1, COMP>(A, X, ldx, Y, ldy, nrhs, alpha, beta, st);
         break;
     case 2: launch_t<VT, BT, IT, RT, 2, COMP>(A, X, ldx, Y, ldy, nrhs, alpha, beta, st); break;
@@ -916,17 +1135,18 @@ void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int6
               int nrhs, double alpha, double beta, hipStream_t st)
 {
     const Balance &Bal = L.bal;
+    const bool cs = L.coo_single;
     const bool b16 = L.bcol_bytes == 2;
     const int s = A.s;
     if (s <= 8) {
-        if (b16) launch_r<VT, uint16_t, uint8_t, uint8_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
-        else launch_r<VT, uint32_t, uint8_t, uint8_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        if (b16) launch_r<VT, uint16_t, uint8_t, uint8_t, COMP>(R, A, Bal, cs, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        else launch_r<VT, uint32_t, uint8_t, uint8_t, COMP>(R, A, Bal, cs, X, ldx, Y, ldy, nrhs, alpha, beta, st);
     } else if (s <= 16) {
-        if (b16) launch_r<VT, uint16_t, uint8_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
-        else launch_r<VT, uint32_t, uint8_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        if (b16) launch_r<VT, uint16_t, uint8_t, uint16_t, COMP>(R, A, Bal, cs, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        else launch_r<VT, uint32_t, uint8_t, uint16_t, COMP>(R, A, Bal, cs, X, ldx, Y, ldy, nrhs, alpha, beta, st);
     } else {
-        if (b16) launch_r<VT, uint16_t, uint16_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
-        else launch_r<VT, uint32_t, uint16_t, uint16_t, COMP>(R, A, Bal, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        if (b16) launch_r<VT, uint16_t, uint16_t, uint16_t, COMP>(R, A, Bal, cs, X, ldx, Y, ldy, nrhs, alpha, beta, st);
+        else launch_r<VT, uint32_t, uint16_t, uint16_t, COMP>(R, A, Bal, cs, X, ldx, Y, ldy, nrhs, alpha, beta, st);
     }
 }
 
@@ -934,17 +1154,35 @@ void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int6
 
 int batch_code(int bcol_bytes, int s) { return (bcol_bytes == 4 ? 3 : 0) + (s <= 8 ? 0 : (s <= 16 ? 1 : 2)); }
 
-void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t total_items, const uint32_t *dheavy,
-                int64_t nheavy, double alpha, double beta, int grid, hipStream_t st, unsigned long long *cycles)
+template <typename BT, typename IT, typename RT>
+void launch_batch_code(const BatchEntry *dE, const uint32_t *item_mat, int64_t g0, int64_t g1, double alpha,
+                       double beta, hipStream_t st, unsigned long long *cycles)
 {
-    if (total_items > 0) {
-        if (cycles)
-            hipLaunchKernelGGL(k_batch_spmv<true>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat,
-                               total_items, alpha, beta, cycles);
-        else
-            hipLaunchKernelGGL(k_batch_spmv<false>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat,
-                               total_items, alpha, beta, cycles);
-        ABHSF_HIP(hipGetLastError());
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((g1 - g0 + 3) / 4, 8192));
+    if (cycles)
+        hipLaunchKernelGGL((k_batch_spmv<BT, IT, RT, true>), dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat,
+                           g0, g1, alpha, beta, cycles);
+    else
+        hipLaunchKernelGGL((k_batch_spmv<BT, IT, RT, false>), dim3((unsigned)grid), dim3(256), 0, st, dE,
+                           item_mat, g0, g1, alpha, beta, cycles);
+    ABHSF_HIP(hipGetLastError());
+}
+
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, const int64_t *code_begin,
+                const int64_t *code_end, const uint32_t *dheavy, int64_t nheavy, double alpha, double beta,
+                hipStream_t st, unsigned long long *cycles)
+{
+    for (int c = 0; c < kBatchCodes; c++) {
+        const int64_t g0 = code_begin[c], g1 = code_end[c];
+        if (g1 <= g0) continue;
+        switch (c) {  // batch_code: (bcol u32 ? 3 : 0) + (s <= 8 ? 0 : s <= 16 ? 1 : 2)
+        case 0: launch_batch_code<uint16_t, uint8_t, uint8_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
+        case 1: launch_batch_code<uint16_t, uint8_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
+        case 2: launch_batch_code<uint16_t, uint16_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
+        case 3: launch_batch_code<uint32_t, uint8_t, uint8_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
+        case 4: launch_batch_code<uint32_t, uint8_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
+        default: launch_batch_code<uint32_t, uint16_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
+        }
     }
     if (nheavy > 0) {
         if (cycles)
@@ -957,10 +1195,38 @@ void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t total_it
     }
 }
 
+void spmv_transposed(const SpmvLaunch &L, const DevMat &A, int nrhs, double scale, const double *X, int64_t ldx,
+                     double *Y, int64_t ldy, hipStream_t st)
+{
+    ABHSF_REQUIRE(L.val_bytes == 8, ABHSF_ERR_UNSUPPORTED, "symmetric storage needs FP64 values");
+    if (A.M == 0) return;
+    Balance b = L.bal;
+    if (!b.light && b.n_units == 0) b.n_light = A.M;
+    const int64_t nblk = (b.n_light + b.n_units + 3) / 4;
+    ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many strips");
+    const bool b16 = L.bcol_bytes == 2;
+    for (int j = 0; j < nrhs; j++) {
+        const double *x = X + j;
+        double *y = Y + j;
+#define ABHSF_TRANS(BT_, IT_, RT_)                                                                             \
+    hipLaunchKernelGGL((k_spmv_trans<BT_, IT_, RT_>), dim3((unsigned)nblk), dim3(256), 0, st, A, b, x, ldx, y, \
+                       ldy, scale)
+        if (A.s <= 8) {
+            if (b16) ABHSF_TRANS(uint16_t, uint8_t, uint8_t); else ABHSF_TRANS(uint32_t, uint8_t, uint8_t);
+        } else if (A.s <= 16) {
+            if (b16) ABHSF_TRANS(uint16_t, uint8_t, uint16_t); else ABHSF_TRANS(uint32_t, uint8_t, uint16_t);
+        } else {
+            if (b16) ABHSF_TRANS(uint16_t, uint16_t, uint16_t); else ABHSF_TRANS(uint32_t, uint16_t, uint16_t);
+        }
+#undef ABHSF_TRANS
+        ABHSF_HIP(hipGetLastError());
+    }
+}
+
 int max_rhs_tile(int s, bool comp)
 {
     int R = 32;
-    while (R > 1 && (size_t)s * R * sizeof(double) * (comp ? 2 : 1) + 2 * kWave * sizeof(uint32_t) > 65536) R >>= 1;
+    while (R > 1 && spmm_wave_lds(s, R, comp) > 65536) R >>= 1;
     return R;
 }
 

@@ -13,7 +13,7 @@ struct Balance {
     const uint32_t *light = nullptr;  // light strip ids (null: every strip, in order)
     const uint32_t *pack = nullptr;   // [n_light + 1] light item k = light[pack[k] .. pack[k+1])
                                       // (null: one strip per light item)
-    const uint32_t *units = nullptr;  // [n_units][4] {scheme, b0, b1, value offset of b0}
+    const uint32_t *units = nullptr;  // [n_units][4] {scheme | strip << 2, b0, b1, value offset of b0}
     const uint32_t *heavy = nullptr;  // [n_heavy][3] {strip, first unit, unit count}
     double *part = nullptr;           // [n_units][s] (x2 when compensated)
     // Column-panel variant: slot table, 4 slots (waves) per workgroup; slot
@@ -27,8 +27,9 @@ constexpr int kXcdCount = 8;            // XCDs of an MI355X; workgroups go roun
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 
 struct SpmvLaunch {
-    int bcol_bytes;  // 2 or 4
-    int val_bytes;   // 8 (f64) or 4 (f32, compensated accumulation)
+    int bcol_bytes;          // 2 or 4
+    int val_bytes;           // 8 (f64) or 4 (f32, compensated accumulation)
+    bool coo_single = false; // COO blocks hold about one entry on average
     Balance bal;
 };
 
@@ -42,15 +43,24 @@ struct BatchEntry {
     int32_t code;   // index-type combination, batch_code()
 };
 
+// Index-type combination of a batch member (block-column width x in-block
+// index widths), 0 .. kBatchCodes - 1.
+constexpr int kBatchCodes = 6;
 int batch_code(int bcol_bytes, int s);
 
-// All matrices' work items in one grid-stride launch, then the heavy-strip
+// All matrices' work items, one grid-stride launch per index-type combination
+// c over global items [code_begin[c], code_end[c]), then the heavy-strip
 // fix-up (dheavy: pairs {matrix, heavy index}).  item_mat[g]: matrix of
 // global item g.  FP64 storage only.  With cycles (device, [nmat], zeroed by
 // the caller) every wave adds its shader-clock cycles to its matrix's counter.
-void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t total_items, const uint32_t *dheavy,
-                int64_t nheavy, double alpha, double beta, int grid, hipStream_t st,
-                unsigned long long *cycles = nullptr);
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, const int64_t *code_begin,
+                const int64_t *code_end, const uint32_t *dheavy, int64_t nheavy, double alpha, double beta,
+                hipStream_t st, unsigned long long *cycles = nullptr);
+
+// Symmetric storage, second pass: Y[j] += scale * T_off^T * X[j] for the
+// stored triangle T without its diagonal (global atomics; FP64 values).
+void spmv_transposed(const SpmvLaunch &L, const DevMat &A, int nrhs, double scale, const double *X, int64_t ldx,
+                     double *Y, int64_t ldy, hipStream_t st);
 
 // Largest power-of-two rhs tile whose s x R accumulator fits the LDS budget.
 int max_rhs_tile(int s, bool comp);

@@ -48,8 +48,9 @@ static void read_mtx(const std::string &fn, elements_t &el, matrix_properties &p
     if (!pattern && line.find("real") == std::string::npos && line.find("integer") == std::string::npos)
         throw std::runtime_error("read_matrix_real_or_binary() invoked for non-real-non-binary matrix.");
     props.type = pattern ? matrix_type_t::BINARY : matrix_type_t::REAL;
-    props.symmetry = line.find("general") != std::string::npos ? matrix_symmetry_t::UNSYMMETRIC
-                                                               : matrix_symmetry_t::SYMMETRIC;
+    props.symmetry = line.find("general") != std::string::npos          ? matrix_symmetry_t::UNSYMMETRIC
+                     : line.find("skew-symmetric") != std::string::npos ? matrix_symmetry_t::SKEW_SYMMETRIC
+                                                                        : matrix_symmetry_t::SYMMETRIC;
     while (std::getline(f, line))
         if (!line.empty() && line[0] != '%') break;
     std::istringstream hs(line);
@@ -126,9 +127,16 @@ int main(int argc, char *argv[])
         HIPCHECK(hipMemcpy(x, hx.data(), props.n * sizeof(double), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(y, hy.data(), props.m * sizeof(double), hipMemcpyHostToDevice));
 
-        // verification (spmv/unified/spmv.cpp:1195-1199)
+        // verification (spmv/unified/spmv.cpp:1195-1199) over all nnz_all
+        // entries: symmetric storage applies the mirrored triangle as well
+        // (the reference's kernels use the stored triangle only)
+        const bool sym = props.symmetry != matrix_symmetry_t::UNSYMMETRIC;
+        const double mirror = props.symmetry == matrix_symmetry_t::SKEW_SYMMETRIC ? -1.0 : 1.0;
         std::vector<double> yt(props.m, 0.0);
-        for (const auto &e : elements) yt[std::get<0>(e)] += std::get<2>(e) * x_element;
+        for (const auto &e : elements) {
+            yt[std::get<0>(e)] += std::get<2>(e) * x_element;
+            if (sym && std::get<0>(e) != std::get<1>(e)) yt[std::get<1>(e)] += mirror * std::get<2>(e) * x_element;
+        }
         double expected = 0.0;
         for (double v : yt) expected += v * v;
         std::cout << "Expected result = " << std::sqrt(expected) << std::endl;

@@ -46,6 +46,10 @@ struct Storage {
 void build_from_device_csr(Storage &S, const int64_t *rowptr, const int32_t *col, const double *val,
                            hipStream_t st);
 
+// Triangle flags of a device CSR matrix (sorted columns): bit 0 an entry
+// below the diagonal, bit 1 above, bit 2 on it.
+uint32_t csr_triangle_flags(const int64_t *rowptr, const int32_t *col, int64_t m, hipStream_t st);
+
 // Recreate S (shape and options already set) from exported arrays in host
 // memory; throws abhsf::Error if the arrays are not a valid storage.
 void import_storage(Storage &S, int32_t count, const char *const *names, const void *const *data,

@@ -43,6 +43,11 @@ class matrix_properties:  # include/abhsf/utils/matrix_properties.h:24-71
     symmetry: matrix_symmetry_t = matrix_symmetry_t.UNSYMMETRIC
 
 
+_SYM = {matrix_symmetry_t.UNSYMMETRIC: capi.GENERAL, matrix_symmetry_t.SYMMETRIC: capi.SYMMETRIC,
+        matrix_symmetry_t.HERMITIAN: capi.SYMMETRIC, matrix_symmetry_t.SKEW_SYMMETRIC: capi.SKEW_SYMMETRIC,
+        "general": capi.GENERAL, "symmetric": capi.SYMMETRIC, "skew": capi.SKEW_SYMMETRIC}
+
+
 def _addr(a) -> int:
     if isinstance(a, hip.DeviceArray):
         return a.address
@@ -80,13 +85,19 @@ class AbhsfMatrix:
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         cols = np.ascontiguousarray(cols, dtype=np.int32)
         vals = None if vals is None else np.ascontiguousarray(vals, dtype=np.float64)
+        # symmetric / skew-symmetric: one stored triangle, SpMV applies both
+        capi.call("abhsf_set_symmetry", self._h, _SYM[props.symmetry])
         capi.call("abhsf_build_coo", self._h, props.m, props.n, rows.size,
                   rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
                   None if vals is None else vals.ctypes.data_as(C.c_void_p))
         self.props = props
 
-    def from_csr(self, m: int, n: int, rowptr, col, val, on_device: bool = False, nnz: int = None) -> None:
-        """Build from CSR (host numpy arrays, or device arrays with on_device)."""
+    def from_csr(self, m: int, n: int, rowptr, col, val, on_device: bool = False, nnz: int = None,
+                 symmetry="general") -> None:
+        """Build from CSR (host numpy arrays, or device arrays with on_device).
+        symmetry "symmetric" / "skew" (or a matrix_symmetry_t): the CSR holds
+        one triangle and SpMV applies the whole matrix."""
+        capi.call("abhsf_set_symmetry", self._h, _SYM[symmetry])
         if on_device:
             if nnz is None:
                 nnz = col.shape[0]
@@ -99,7 +110,9 @@ class AbhsfMatrix:
             capi.call("abhsf_build_csr", self._h, m, n, int(rowptr[-1]),
                       rowptr.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p),
                       None if val is None else val.ctypes.data_as(C.c_void_p), 0)
-        self.props = matrix_properties(m, n, self.stats()["nnz"])
+        sym = {capi.GENERAL: matrix_symmetry_t.UNSYMMETRIC, capi.SYMMETRIC: matrix_symmetry_t.SYMMETRIC,
+               capi.SKEW_SYMMETRIC: matrix_symmetry_t.SKEW_SYMMETRIC}[_SYM[symmetry]]
+        self.props = matrix_properties(m, n, self.stats()["nnz"], symmetry=sym)
 
     # -- apply ---------------------------------------------------------------
     def spmv(self, x, y) -> None:

@@ -283,8 +283,16 @@ class Model:
                    "sd": list(map(float, self.sd)), "ymu": self.ymu, "meta": self.meta},
                   open(path, "w"))
 
+    def compatible(self) -> bool:
+        """True when the model was trained on this feature layout (NFEAT inputs)."""
+        return self.mu.shape == (NFEAT,)
+
     def normalize(self, X):
-        return ((np.asarray(X, dtype=np.float64) - self.mu) / self.sd).astype(np.float32)
+        X = np.asarray(X, dtype=np.float64)
+        if X.shape[-1] != self.mu.shape[0]:
+            raise ValueError(f"model expects {self.mu.shape[0]} features, got {X.shape[-1]}: retrain it "
+                             "(python -m abhsf_amd.train_tuner)")
+        return ((X - self.mu) / self.sd).astype(np.float32)
 
     def predict(self, X):
         """log(ms per entry) predictions, computed by the GPU forward kernel."""

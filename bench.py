@@ -143,6 +143,9 @@ def tuned_pick(A, allowed_s, hip):
     if not os.path.exists(tuner.MODEL_PATH):
         return None, "none"
     model = tuner.Model.load()
+    if not model.compatible():
+        log("tuner model does not match the feature layout: exhaustive timing decides")
+        return None, "none"
     Xc, _ = tuner.matrix_features(A)
     pred = model.predict(Xc)
     cands = tuner.candidates()
@@ -351,6 +354,9 @@ def run_batch(args, hip):
     stream = hip.Stream()
     mpath = tuner.BATCH_MODEL_PATH if os.path.exists(tuner.BATCH_MODEL_PATH) else tuner.MODEL_PATH
     model = tuner.Model.load(mpath) if os.path.exists(mpath) else None
+    if model is not None and not model.compatible():
+        log("tuner model does not match the feature layout: fixed s=64")
+        model = None
     tuned, plain, xs, ys, nnz, indiv, picks = [], [], [], [], 0, [], {}
     t0 = time.perf_counter()
     for i in range(args.batch_count):

@@ -71,6 +71,11 @@ class abhsf_matrix {
             c[k] = std::get<1>(elements[k]);
             v[k] = std::get<2>(elements[k]);
         }
+        // symmetric / skew-symmetric: one stored triangle, SpMV applies both
+        const int32_t sym = props.symmetry == matrix_symmetry_t::SKEW_SYMMETRIC ? ABHSF_SKEW_SYMMETRIC
+                            : props.symmetry == matrix_symmetry_t::UNSYMMETRIC  ? ABHSF_GENERAL
+                                                                                : ABHSF_SYMMETRIC;
+        check(abhsf_set_symmetry(h_, sym), "abhsf_set_symmetry");
         check(abhsf_build_coo(h_, (int64_t)props.m, (int64_t)props.n, (int64_t)elements.size(), r.data(), c.data(),
                               v.data()),
               "abhsf_build_coo");

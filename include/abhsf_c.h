@@ -73,6 +73,15 @@ enum {
                                    that XCD's L2; strips spanning panels are split there */
 };
 
+/* Symmetry of the stored matrix (matrix_symmetry_t, include/abhsf/utils/
+ * matrix_properties.h; Hermitian equals symmetric for real values).  With
+ * SYMMETRIC / SKEW_SYMMETRIC only one triangle is stored (the entries given
+ * to the build, which must all lie on one side of the diagonal; no diagonal
+ * for skew) and SpMV/SpMM apply the whole matrix: A = T + T_off^T or
+ * T - T^T, so a stored triangle of nnz_stored entries does the work of
+ * nnz_all (spmv/unified/spmv.cpp:1160-1166).  FP64 values only. */
+enum { ABHSF_GENERAL = 0, ABHSF_SYMMETRIC = 1, ABHSF_SKEW_SYMMETRIC = 2 };
+
 typedef struct abhsf_options {
     int32_t block_size;   /* s: power of two in [2, 256] */
     int32_t policy;       /* ABHSF_POLICY_* */
@@ -121,6 +130,13 @@ abhsf_status abhsf_build_coo(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz,
 
 /* Storage statistics (blocks per scheme, footprints under both cost models). */
 abhsf_status abhsf_get_stats(const abhsf_handle *h, abhsf_stats *out);
+
+/* Symmetry (ABHSF_GENERAL / _SYMMETRIC / _SKEW_SYMMETRIC) for the next build of
+ * h; the reference carries it in matrix_properties::symmetry, passed to
+ * from_elements (spmv/unified/spmv.cpp:74).  Building symmetric storage from
+ * entries in both triangles fails with the reference reader's message
+ * (matrix_market_reader.h:191-197). */
+abhsf_status abhsf_set_symmetry(abhsf_handle *h, int32_t symmetry);
 
 /* y = alpha * A * x + beta * y (device pointers; beta == 0 does not read y).
  * With alpha = beta = 1 this is the reference's accumulating

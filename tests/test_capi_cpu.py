@@ -69,3 +69,20 @@ def test_package_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(capi, "_lib", None)
     with pytest.raises(OSError):
         capi.lib()
+
+
+def test_symmetry_argument_checks():
+    """Checked before any GPU work: unknown symmetry, non-square and FP32
+    symmetric storage are refused."""
+    import numpy as np
+    from abhsf_amd import AbhsfError, AbhsfMatrix
+    M = AbhsfMatrix(block_size=16)
+    with pytest.raises(AbhsfError):
+        capi.call("abhsf_set_symmetry", M._h, 7)
+    rp = np.array([0, 1, 1], dtype=np.int64)
+    col = np.array([0], dtype=np.int32)
+    with pytest.raises(AbhsfError, match="square"):
+        M.from_csr(2, 3, rp, col, np.ones(1), symmetry="symmetric")
+    M32 = AbhsfMatrix(block_size=16, precision="f32")
+    with pytest.raises(AbhsfError, match="FP64"):
+        M32.from_csr(2, 2, rp, col, np.ones(1), symmetry="symmetric")
