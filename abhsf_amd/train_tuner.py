@@ -18,7 +18,8 @@ Writes DIR/tuner_model.json, DIR/tuner_dataset.npz and DIR/tuner_report.json.
 
 Mode batch (many matrices in one launch): matrices drawn like the batch
 workload (synth.batch_member, other seeds: 300 train, 100 held-out); for each
-candidate every member is built at that candidate and one batched launch is
+candidate (s, balanced variant: tuner.BATCH_VARIANTS) every member is built at
+that candidate and one batched launch is
 profiled (abhsf_batch_profile): the label is log(cycles per entry) of the
 member's work items inside the full launch, i.e. its share of the GPU.  The
 held-out report gives per-matrix regret and the measured batched launch time
@@ -125,10 +126,11 @@ def collect_batch(split, stream, log):
     """Features and in-launch cycle labels of every member for every candidate."""
     t0 = time.perf_counter()
     mats = batch_corpus(split)
-    cands = tuner.candidates()
+    cands = tuner.candidates(tuner.BATCH_VARIANTS)
+    rows = [tuner.candidates().index(c) for c in cands]
     X = np.zeros((len(mats), len(cands), tuner.NFEAT))
     for i, A in enumerate(mats):
-        X[i] = tuner.matrix_features(A)[0]
+        X[i] = tuner.matrix_features(A)[0][rows]
     log(f"{split}: {len(mats)} matrices, {sum(A.nnz for A in mats)} nnz, features "
         f"({time.perf_counter() - t0:.0f} s)")
     xs = [hip.DeviceArray.from_numpy(np.random.default_rng(i).uniform(-1, 1, A.n)) for i, A in enumerate(mats)]
@@ -151,7 +153,7 @@ def collect_batch(split, stream, log):
 
 def main_batch(args, log):
     stream = hip.Stream()
-    cands = tuner.candidates()
+    cands = tuner.candidates(tuner.BATCH_VARIANTS)
     _, Xtr, ctr, ntr, gtr, _, _ = collect_batch("train", stream, log)
     Xf = Xtr.reshape(-1, tuner.NFEAT)
     ytr = np.log(np.maximum(ctr, 1.0) / ntr[:, None]).reshape(-1)
@@ -159,6 +161,7 @@ def main_batch(args, log):
     model, losses = tuner.train(Xf, ytr, w=w, steps=args.steps, lr=args.lr)
     log(f"trained: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {model.meta['train_s']:.1f} s")
     model.meta["labels"] = "log(cycles per entry inside a batched launch)"
+    model.meta["variants"] = list(tuner.BATCH_VARIANTS)
     model.save(os.path.join(args.out, "tuner_model_batch.json"))
     mats, Xte, cte, nte, gte, xs, ys = collect_batch("test", stream, log)
     np.savez(os.path.join(args.out, "tuner_dataset_batch.npz"), Xtr=Xtr, ctr=ctr, ntr=ntr, Xte=Xte, cte=cte,

@@ -156,11 +156,17 @@ def feature_vector(g, sf, variant):
     ], dtype=np.float64)
 
 
-NFEAT = 6 + 16
+NFEAT = 6 + 17  # 6 global + 17 per candidate (fractions are 4 values each)
 
 
-def candidates():
-    return [(s, v) for s in S_CANDIDATES for v in VARIANTS]
+# Batched launches use the balanced variant only: inside one launch a whole
+# heavy strip on one wave (strip variant) sets the launch's tail, which the
+# per-member cycle label cannot see, and column panels only add partial rows.
+BATCH_VARIANTS = ("balanced",)
+
+
+def candidates(variants=VARIANTS):
+    return [(s, v) for s in S_CANDIDATES for v in VARIANTS if v in variants]
 
 
 def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
@@ -331,9 +337,14 @@ def train(X, y, w=None, steps=3000, lr=3e-3, seed=0, mu=None, sd=None):
 
 
 def pick(model, Xc):
-    """Candidate with the smallest predicted time, and the predictions."""
+    """Candidate with the smallest predicted time among the variants the model
+    was trained on (Xc: feature rows of every candidate), and the
+    predictions."""
+    allowed = model.meta.get("variants", VARIANTS)
     pred = model.predict(Xc)
-    return candidates()[int(np.argmin(pred))], pred
+    cands = candidates()
+    ok = [i for i, (_, v) in enumerate(cands) if v in allowed]
+    return cands[min(ok, key=lambda i: pred[i])], pred
 
 
 def pick_storage_optimal(per_s):

@@ -238,10 +238,14 @@ def run_block_mix(args, hip):
     log(f"matrix block_mix n={A.n} nnz={A.nnz} checksum={A.checksum():#x} ({time.perf_counter() - t0:.1f} s)")
     stream = hip.Stream()
     s_list = [int(v) for v in (args.s or "16,32,64,128").split(",")]
-    pick, tname = tuned_pick(A, s_list, hip)
-    cands = [(s, "balanced") for s in s_list]
-    if pick is not None and pick not in cands:
-        cands.append(pick)
+    if args.variant:  # fixed configuration (profiling runs): no tuner, nothing else measured
+        pick, tname = (s_list[0], args.variant), "fixed"
+        cands = [pick]
+    else:
+        pick, tname = tuned_pick(A, s_list, hip)
+        cands = [(s, "balanced") for s in s_list]
+        if pick is not None and pick not in cands:
+            cands.append(pick)
     res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
     if pick is None:  # no trained model: exhaustive timing decides
         pick = max(res, key=lambda k: res[k]["gflops"])
@@ -429,6 +433,8 @@ def main():
     ap.add_argument("--workload", default="block_mix", choices=["block_mix", "rmat", "spmm", "batch"])
     ap.add_argument("--s", default=None, help="block sizes to measure (block_mix default 16,32,64,128; "
                     "rmat default all tuner candidates)")
+    ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel"],
+                    help="block_mix: measure only (first --s, VARIANT), no tuner (for profiler passes)")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--spmm-s", type=int, default=64)
     ap.add_argument("--batch-count", type=int, default=2000)

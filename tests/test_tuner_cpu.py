@@ -65,3 +65,13 @@ def test_layout_bytes_matches_oracle_storage(s):
         assert sf["stored_bytes"] == pytest.approx(stored)
         tot = sum(B.nb)
         assert np.allclose(sf["blk_frac"], [v / tot for v in B.nb])
+
+
+def test_feature_vector_length_matches_nfeat():
+    from abhsf_amd import tuner
+    g = {"log2_n": 1.0, "log2_nnz": 1.0, "log2_mean_row": 0.0, "row_cv": 0.0, "log2_max_row": 0.0,
+         "empty_rows": 0.0}
+    sf = {"log2_s": 3.0, "log2_blk_per_entry": 0.0, "blk_frac": [1, 0, 0, 0], "ent_frac": [1, 0, 0, 0],
+          "log2_bytes_per_entry": 3.5, "strip_cv": 0.0, "log2_max_strip": 0.0, "heavy_frac": 0.0, "log2_M": 4.0}
+    for v in tuner.VARIANTS:
+        assert tuner.feature_vector(g, sf, v).shape == (tuner.NFEAT,)
