@@ -7,9 +7,13 @@ value 1.0; for symmetric / skew-symmetric / Hermitian matrices only the stored
 triangle is kept and mixing both triangles is an error (:191-197).
 `expand_symmetric` mirrors the stored triangle into a general matrix for
 y = A*x over all entries (the reference counts those as nnz_all,
-spmv/unified/spmv.cpp:1160-1166).
+spmv/unified/spmv.cpp:1160-1166); without it, build symmetric storage
+(AbhsfMatrix.from_elements with the returned props).  read_mm / write_mm: the
+binary layout of utils/mtx2mm/mtx2mm.cpp:101-136 (cached conversions).
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -38,7 +42,7 @@ def read(path: str, expand_symmetric: bool = False):
             raise MatrixMarketError("Unsupported matrix type.")
         if sym not in syms:
             raise MatrixMarketError("Unsupported type of matrix symmetry.")
-        if field == "complex":
+        if field not in ("real", "pattern"):  # matrix_market_reader.h:155-156
             raise MatrixMarketError("read_matrix_real_or_binary() invoked for non-real-non-binary matrix.")
         line = f.readline()
         while line.startswith("%"):
@@ -56,19 +60,27 @@ def read(path: str, expand_symmetric: bool = False):
     vals = np.ones(nnz) if field == "pattern" else data[:, 2].astype(np.float64)
     if nnz and (rows.min() < 0 or cols.min() < 0 or rows.max() >= m or cols.max() >= n):
         raise MatrixMarketError("Matrix element index out of range.")
-    keep = vals != 0.0
-    rows, cols, vals = rows[keep], cols[keep], vals[keep]
-    props = matrix_properties(m, n, int(rows.size), types[field], syms[sym])
-    off = rows != cols
-    if props.symmetry != matrix_symmetry_t.UNSYMMETRIC:
+    return _finish(m, n, rows, cols, vals, types[field], syms[sym], expand_symmetric, path)
+
+
+def _finish(m, n, rows, cols, vals, mtype, msym, expand_symmetric, name):
+    """Zeros dropped, one stored triangle for symmetric kinds (the triangle
+    test sees every stored element, zeros included:
+    matrix_market_reader.h:163-197)."""
+    if msym != matrix_symmetry_t.UNSYMMETRIC:
         if np.any(rows > cols) and np.any(cols > rows):
             raise MatrixMarketError("Elements from both L and U parts stored for not-unsymmetric matrix.")
+    if mtype == matrix_type_t.REAL:
+        keep = vals != 0.0
+        rows, cols, vals = rows[keep], cols[keep], vals[keep]
+    props = matrix_properties(m, n, int(rows.size), mtype, msym)
+    off = rows != cols
     nnz_all = int(rows.size + (off.sum() if props.symmetry != matrix_symmetry_t.UNSYMMETRIC else 0))
     if expand_symmetric and props.symmetry != matrix_symmetry_t.UNSYMMETRIC:
         sign = -1.0 if props.symmetry == matrix_symmetry_t.SKEW_SYMMETRIC else 1.0
         rows, cols, vals = (np.concatenate([rows, cols[off]]), np.concatenate([cols, rows[off]]),
                             np.concatenate([vals, sign * vals[off]]))
-    return from_triplets(m, n, rows, cols, vals, path), props, nnz_all
+    return from_triplets(m, n, rows, cols, vals, name), props, nnz_all
 
 
 def write(path: str, A: Csr, field: str = "real") -> None:
@@ -81,3 +93,52 @@ def write(path: str, A: Csr, field: str = "real") -> None:
         else:
             for a, b, x in zip(r, c, v):
                 f.write(f"{a + 1} {b + 1} {float(x)!r}\n")
+
+
+# ---- binary .mm layout of utils/mtx2mm/mtx2mm.cpp:101-136 --------------------
+# u64 m, n, nnz, type (matrix_type_t: BINARY 0, INTEGER 1, REAL 2, COMPLEX 3),
+# symmetry (matrix_symmetry_t: SYMMETRIC 0, HERMITIAN 1, SKEW_SYMMETRIC 2,
+# UNSYMMETRIC 3); u32 rows[nnz], u32 cols[nnz] (0-based, as read); values:
+# i64[nnz] (INTEGER), f64[nnz] (REAL), f64 re[nnz] + f64 im[nnz] (COMPLEX),
+# none (BINARY).  Little-endian, no padding.
+
+def write_mm(path: str, A: Csr, props: matrix_properties = None) -> None:
+    """Write a real (or, with props.type BINARY, pattern) matrix in the
+    binary layout; A holds the elements as stored (one triangle for
+    symmetric props)."""
+    props = props or matrix_properties(A.m, A.n, A.nnz)
+    r, c, v = A.triplets()
+    with open(path, "wb") as f:
+        np.array([A.m, A.n, A.nnz, int(props.type), int(props.symmetry)], dtype="<u8").tofile(f)
+        r.astype("<u4").tofile(f)
+        c.astype("<u4").tofile(f)
+        if props.type == matrix_type_t.REAL:
+            np.asarray(v, dtype="<f8").tofile(f)
+        elif props.type != matrix_type_t.BINARY:
+            raise MatrixMarketError("write_mm writes real or pattern matrices")
+
+
+def read_mm(path: str, expand_symmetric: bool = False):
+    """Read the binary layout with read()'s semantics (real or pattern only,
+    zeros dropped, one triangle for symmetric kinds); returns (Csr,
+    matrix_properties, nnz_all)."""
+    size = os.path.getsize(path)
+    with open(path, "rb") as f:
+        hdr = np.fromfile(f, dtype="<u8", count=5)
+        if hdr.size != 5:
+            raise MatrixMarketError("binary matrix header truncated")
+        m, n, nnz, mtype, msym = (int(x) for x in hdr)
+        if mtype > 3 or msym > 3:
+            raise MatrixMarketError("binary matrix header holds an unknown type or symmetry")
+        mtype, msym = matrix_type_t(mtype), matrix_symmetry_t(msym)
+        if mtype not in (matrix_type_t.REAL, matrix_type_t.BINARY):
+            raise MatrixMarketError("read_matrix_real_or_binary() invoked for non-real-non-binary matrix.")
+        want = 40 + 8 * nnz + (8 * nnz if mtype == matrix_type_t.REAL else 0)
+        if size != want:
+            raise MatrixMarketError(f"binary matrix size {size} differs from the header's {want}")
+        rows = np.fromfile(f, dtype="<u4", count=nnz).astype(np.int64)
+        cols = np.fromfile(f, dtype="<u4", count=nnz).astype(np.int64)
+        vals = np.fromfile(f, dtype="<f8", count=nnz) if mtype == matrix_type_t.REAL else np.ones(nnz)
+    if nnz and (rows.max() >= m or cols.max() >= n):
+        raise MatrixMarketError("Matrix element index out of range.")
+    return _finish(m, n, rows, cols, vals, mtype, msym, expand_symmetric, path)
