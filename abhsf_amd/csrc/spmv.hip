@@ -517,11 +517,6 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
                                             s_st[wave], lane);
 }
 
-// Many matrices in one launch: the waves take the global items [g0, g1) of
-// one index-type combination, wave w items g0 + w, g0 + w + W, ... (W = all
-// waves of the grid); item_mat[g] names the matrix of item g.  FP64 values.
-// TIMED: each wave also adds the shader-clock cycles of its item to
-// cyc[matrix] (the per-matrix cost inside a full launch).
 // ===========================================================================
 // Symmetric storage (one triangle T stored): the main pass computes
 // y = alpha*T*x + beta*y, this pass adds scale * T_off^T * x (scale = alpha,
@@ -663,6 +658,11 @@ __device__ inline T load_uniform(const T *p)
     return out;
 }
 
+// Many matrices in one launch: the waves take the global items [g0, g1) of
+// one index-type combination, wave w items g0 + w, g0 + w + W, ... (W = all
+// waves of the grid); item_mat[g] names the matrix of item g.  FP64 values.
+// TIMED: each wave also adds the shader-clock cycles of its item to
+// cyc[matrix] (the per-matrix cost inside a full launch).
 template <typename BT, typename IT, typename RT, bool TIMED>
 __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict__ E,
                                                     const uint32_t *__restrict__ item_mat, int64_t g0, int64_t g1,

@@ -284,10 +284,14 @@ def run_rmat(args, hip):
     log(f"matrix rmat scale 22 ef 16 nnz={A.nnz}")
     stream = hip.Stream()
     s_allowed = tuner.S_CANDIDATES if args.s is None else [int(v) for v in args.s.split(",")]
-    pick, tname = tuned_pick(A, s_allowed, hip)
-    cands = [c for c in tuner.candidates() if c[0] in s_allowed]
-    if args.s is not None:  # a restricted run (profiling): the balanced variant only, plus the pick
-        cands = [c for c in cands if c[1] == "balanced" or c == pick]
+    if args.variant:  # fixed configuration (profiling runs): no tuner, nothing else measured
+        pick, tname = (s_allowed[0], args.variant), "fixed"
+        cands = [pick]
+    else:
+        pick, tname = tuned_pick(A, s_allowed, hip)
+        cands = [c for c in tuner.candidates() if c[0] in s_allowed]
+        if args.s is not None:  # a restricted run: the balanced variant only, plus the pick
+            cands = [c for c in cands if c[1] == "balanced" or c == pick]
     res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
     if pick is None:
         pick, tname = max(res, key=lambda k: res[k]["gflops"]), "exhaustive-timing"
@@ -300,7 +304,7 @@ def run_rmat(args, hip):
     out["roofline"] = roofline(best, f"rmat/{pick[0]}/{pick[1]}")
     out["plain_s64"] = {"gflops": plain["gflops"], "gbs": plain["gbs"]}
     out["tuned_vs_plain"] = round(best["gflops"] / plain["gflops"], 3)
-    out["regret_pct"] = round(tuner.regret(times, pick), 2)
+    out["regret_pct"] = round(tuner.regret(times, pick), 2) if len(times) > 1 else None
     out["per_candidate"] = {f"{s}/{v}": r for (s, v), r in res.items()}
     if not args.no_calib:
         out["copy_calibration"] = calib_copy_gbs(hip)
@@ -439,7 +443,7 @@ def main():
     ap.add_argument("--s", default=None, help="block sizes to measure (block_mix default 16,32,64,128; "
                     "rmat default all tuner candidates)")
     ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel"],
-                    help="block_mix: measure only (first --s, VARIANT), no tuner (for profiler passes)")
+                    help="block_mix / rmat: measure only (first --s, VARIANT), no tuner (profiler passes)")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--spmm-s", type=int, default=64)
     ap.add_argument("--batch-count", type=int, default=2000)
