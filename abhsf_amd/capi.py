@@ -27,6 +27,8 @@ EXPORTED = (
     "abhsf_mlp_backward", "abhsf_mlp_train", "abhsf_batch_create", "abhsf_batch_set_vectors",
     "abhsf_batch_spmv", "abhsf_batch_synchronize", "abhsf_batch_profile", "abhsf_batch_destroy", "abhsf_synchronize",
     "abhsf_destroy", "abhsf_last_error", "abhsf_version", "abhsf_set_symmetry",
+    "abhsf_delta_create", "abhsf_delta_build_csr", "abhsf_delta_spmv", "abhsf_delta_get_stats",
+    "abhsf_delta_export", "abhsf_delta_synchronize", "abhsf_delta_destroy",
 )
 
 
@@ -71,6 +73,18 @@ class AbhsfError(RuntimeError):
         self.detail = msg
 
 
+class DeltaStats(C.Structure):
+    _fields_ = [
+        ("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64), ("key_bits", C.c_int32), ("chunk", C.c_int32),
+        ("words", C.c_uint64 * 4), ("stored_bytes", C.c_uint64), ("build_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"m": self.m, "n": self.n, "nnz": self.nnz, "key_bits": self.key_bits, "chunk": self.chunk,
+                "words": {w: int(self.words[i]) for i, w in enumerate((1, 2, 4, 8))},
+                "stored_bytes": int(self.stored_bytes), "build_ms": float(self.build_ms)}
+
+
 _lib = None
 
 
@@ -101,6 +115,13 @@ def lib():
         L.abhsf_batch_synchronize.argtypes = [vp]
         L.abhsf_batch_profile.argtypes = [vp, d, d, vp]
         L.abhsf_set_symmetry.argtypes = [vp, i32]
+        L.abhsf_delta_create.argtypes = [vp, C.POINTER(vp)]
+        L.abhsf_delta_build_csr.argtypes = [vp, i64, i64, i64, vp, vp, vp, i32]
+        L.abhsf_delta_spmv.argtypes = [vp, d, vp, d, vp]
+        L.abhsf_delta_get_stats.argtypes = [vp, C.POINTER(DeltaStats)]
+        L.abhsf_delta_export.argtypes = [vp, C.c_char_p, vp, sz, C.POINTER(sz)]
+        L.abhsf_delta_synchronize.argtypes = [vp]
+        L.abhsf_delta_destroy.argtypes = [vp]
         L.abhsf_batch_destroy.argtypes = [vp]
         L.abhsf_synchronize.argtypes = [vp]
         L.abhsf_destroy.argtypes = [vp]

@@ -310,6 +310,7 @@ SpmvLaunch Storage::launch() const
     L.bcol_bytes = bcol_bytes;
     L.val_bytes = val_bytes;
     L.coo_single = nb[COO] > 0 && 4 * ne[COO] < 5 * nb[COO];  // mean entries per COO block below 1.25
+    L.has_dense = nb[DENSE] > 0;
     L.bal.light = bal_light.get<uint32_t>();
     L.bal.pack = bal_pack.get<uint32_t>();
     L.bal.units = bal_units.get<uint32_t>();

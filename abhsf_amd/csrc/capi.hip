@@ -1,4 +1,5 @@
 // capi.hip -- the C ABI of libabhsf.so (declared in include/abhsf_c.h).
+#include "delta.hpp"
 #include "storage.hpp"
 
 #include <algorithm>
@@ -30,6 +31,11 @@ struct abhsf_handle {
     abhsf::Storage S;
     int32_t symmetry = ABHSF_GENERAL;  // applies to the next build
     int32_t built_symmetry = ABHSF_GENERAL;
+};
+
+struct abhsf_delta {
+    hipStream_t stream = nullptr;
+    abhsf::DeltaStorage D;
 };
 
 struct abhsf_batch {
@@ -524,6 +530,101 @@ abhsf_status abhsf_batch_destroy(abhsf_batch *b)
         if (b->graph) (void)hipGraphDestroy(b->graph);
         if (b->own_stream) (void)hipStreamDestroy(b->stream);
         delete b;
+    });
+}
+
+abhsf_status abhsf_delta_create(void *stream, abhsf_delta **out)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(out, ABHSF_ERR_INVALID, "null argument");
+        *out = new abhsf_delta();
+        (*out)->stream = static_cast<hipStream_t>(stream);
+    });
+}
+
+abhsf_status abhsf_delta_build_csr(abhsf_delta *d, int64_t m, int64_t n, int64_t nnz, const int64_t *rowptr,
+                                   const int32_t *col, const double *val, int32_t on_device)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(d && rowptr, ABHSF_ERR_INVALID, "null argument");
+        check_shape(m, n, nnz);
+        if (!on_device) {
+            ABHSF_REQUIRE(rowptr[m] == nnz, ABHSF_ERR_INVALID, "row pointers do not span nnz");
+            abhsf::delta_build(d->D, m, n, rowptr, col, val, d->stream);
+            return;
+        }
+        // the encoder runs on the host: bring the device CSR over
+        std::vector<int64_t> rp((size_t)m + 1);
+        std::vector<int32_t> c((size_t)nnz);
+        std::vector<double> v(val ? (size_t)nnz : 0);
+        ABHSF_HIP(hipMemcpyAsync(rp.data(), rowptr, rp.size() * 8, hipMemcpyDeviceToHost, d->stream));
+        if (nnz) ABHSF_HIP(hipMemcpyAsync(c.data(), col, c.size() * 4, hipMemcpyDeviceToHost, d->stream));
+        if (val && nnz) ABHSF_HIP(hipMemcpyAsync(v.data(), val, v.size() * 8, hipMemcpyDeviceToHost, d->stream));
+        ABHSF_HIP(hipStreamSynchronize(d->stream));
+        ABHSF_REQUIRE(rp[(size_t)m] == nnz, ABHSF_ERR_INVALID, "row pointers do not span nnz");
+        abhsf::delta_build(d->D, m, n, rp.data(), c.data(), val ? v.data() : nullptr, d->stream);
+    });
+}
+
+abhsf_status abhsf_delta_spmv(abhsf_delta *d, double alpha, const double *x, double beta, double *y)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(d && x && y, ABHSF_ERR_INVALID, "null argument");
+        abhsf::delta_spmv(d->D, alpha, x, beta, y, d->stream);
+    });
+}
+
+abhsf_status abhsf_delta_get_stats(const abhsf_delta *d, abhsf_delta_stats *out)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(d && out, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(d->D.built, ABHSF_ERR_STATE, "matrix not built");
+        std::memset(out, 0, sizeof(*out));
+        out->m = d->D.m;
+        out->n = d->D.n;
+        out->nnz = d->D.nnz;
+        out->key_bits = d->D.K;
+        out->chunk = abhsf::kDeltaChunk;
+        for (int t = 0; t < 4; t++) out->words[t] = d->D.count[t];
+        out->stored_bytes = d->D.stored_bytes();
+        out->build_ms = d->D.build_ms;
+    });
+}
+
+abhsf_status abhsf_delta_export(const abhsf_delta *d, const char *name, void *dst, size_t cap, size_t *bytes)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(d && name && bytes, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(d->D.built, ABHSF_ERR_STATE, "matrix not built");
+        const std::string nm(name);
+        const abhsf::DevBuf *buf = nm == "tags" ? &d->D.tags : nm == "w8" ? &d->D.words[0]
+                                   : nm == "w16" ? &d->D.words[1] : nm == "w32" ? &d->D.words[2]
+                                   : nm == "w64" ? &d->D.words[3] : nm == "base" ? &d->D.base
+                                   : nm == "start" ? &d->D.start : nm == "val" ? &d->D.val : nullptr;
+        ABHSF_REQUIRE(buf, ABHSF_ERR_INVALID, "unknown array " + nm);
+        *bytes = buf->bytes();
+        if (!dst) return;
+        ABHSF_REQUIRE(cap >= buf->bytes(), ABHSF_ERR_INVALID, "destination too small");
+        if (buf->bytes())
+            ABHSF_HIP(hipMemcpyAsync(dst, buf->get(), buf->bytes(), hipMemcpyDeviceToHost, d->stream));
+        ABHSF_HIP(hipStreamSynchronize(d->stream));
+    });
+}
+
+abhsf_status abhsf_delta_synchronize(abhsf_delta *d)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(d, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_HIP(hipStreamSynchronize(d->stream));
+    });
+}
+
+abhsf_status abhsf_delta_destroy(abhsf_delta *d)
+{
+    return guarded([&] {
+        if (!d) return;
+        (void)hipStreamSynchronize(d->stream);
+        delete d;
     });
 }
 

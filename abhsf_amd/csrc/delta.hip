@@ -1,0 +1,220 @@
+// delta.hip -- delta-coded coordinate storage and its SpMV (the delta codecs
+// of spmv/unified/spmv.cpp:628-1033, re-laid out for wavefront decode).
+//
+// The reference decodes its varint stream with a serial running sum, one
+// thread per contiguous range (:923-944): the width of element k is only
+// known after element k-1 has been read.  Here a wavefront decodes 64
+// elements per step: each lane reads its element's 2-bit tag from the tag
+// words, ballots per width give the element's index in its width stream
+// (so all loads are issued at once), and an inclusive wave scan of the
+// deltas plus the chunk's base key gives every key.  Rows are
+// non-decreasing along the elements, so products are summed per row with a
+// segmented scan and one lane per row segment adds the sum to y atomically
+// (a row may continue into the next step or chunk).
+#include "delta.hpp"
+
+#include <chrono>
+#include <vector>
+
+namespace abhsf {
+namespace {
+
+constexpr int kWaveD = 64;
+constexpr int kStepsD = kDeltaChunk / kWaveD;
+
+__device__ inline uint64_t shfl_up_u64(uint64_t v, int d)
+{
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, kWaveD);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, kWaveD);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint64_t shfl_down_u64(uint64_t v, int d)
+{
+    const uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)v, d, kWaveD);
+    const uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(v >> 32), d, kWaveD);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(256) void k_delta_spmv(const uint32_t *__restrict__ tags,
+                                                    const uint8_t *__restrict__ w8,
+                                                    const uint16_t *__restrict__ w16,
+                                                    const uint32_t *__restrict__ w32,
+                                                    const uint64_t *__restrict__ w64,
+                                                    const uint64_t *__restrict__ base,
+                                                    const uint32_t *__restrict__ start,
+                                                    const double *__restrict__ val, int64_t nnz, int K,
+                                                    int64_t nchunk, const double *__restrict__ x, double *y,
+                                                    double alpha)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 4 + wave;
+    if (c >= nchunk) return;
+    uint64_t carry = base[c];
+    uint32_t o0 = start[c * 4 + 0], o1 = start[c * 4 + 1], o2 = start[c * 4 + 2], o3 = start[c * 4 + 3];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t kmask = (1ull << K) - 1ull;
+#pragma unroll 1
+    for (int step = 0; step < kStepsD; step++) {
+        const int64_t e0 = c * kDeltaChunk + (int64_t)step * kWaveD;
+        if (e0 >= nnz) break;  // wave-uniform
+        const int64_t e = e0 + lane;
+        const bool valid = e < nnz;
+        const uint32_t tag = valid ? (tags[e >> 4] >> (2 * (e & 15))) & 3u : 4u;
+        const uint64_t m0 = __ballot(tag == 0), m1 = __ballot(tag == 1), m2 = __ballot(tag == 2),
+                       m3 = __ballot(tag == 3);
+        uint64_t word = 0;
+        switch (tag) {
+        case 0: word = w8[o0 + (uint32_t)__popcll(m0 & lt)]; break;
+        case 1: word = w16[o1 + (uint32_t)__popcll(m1 & lt)]; break;
+        case 2: word = w32[o2 + (uint32_t)__popcll(m2 & lt)]; break;
+        case 3: word = w64[o3 + (uint32_t)__popcll(m3 & lt)]; break;
+        default: break;
+        }
+        o0 += (uint32_t)__popcll(m0);
+        o1 += (uint32_t)__popcll(m1);
+        o2 += (uint32_t)__popcll(m2);
+        o3 += (uint32_t)__popcll(m3);
+        // keys: inclusive scan of the deltas (invalid lanes add 0)
+        uint64_t key = word >> 2;
+#pragma unroll
+        for (int d = 1; d < kWaveD; d <<= 1) {
+            const uint64_t u = shfl_up_u64(key, d);
+            if (lane >= d) key += u;
+        }
+        key += carry;
+        carry = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(key >> 32), kWaveD - 1) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, kWaveD - 1);
+        const uint64_t row = valid ? key >> K : ~0ull;
+        double p = valid ? alpha * val[e] * x[key & kmask] : 0.0;
+        // segmented inclusive sum over runs of equal rows
+        const uint64_t prev = shfl_up_u64(row, 1), next = shfl_down_u64(row, 1);
+        int f = (lane == 0 || prev != row) ? 1 : 0;
+#pragma unroll
+        for (int d = 1; d < kWaveD; d <<= 1) {
+            const double pu = __shfl_up(p, d, kWaveD);
+            const int fu = __shfl_up(f, d, kWaveD);
+            if (lane >= d && !f) {
+                p += pu;
+                f = fu;
+            }
+        }
+        if (valid && (lane == kWaveD - 1 || next != row)) atomicAdd(&y[row], p);
+    }
+}
+
+__global__ void k_delta_scale(double *y, int64_t m, double beta)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < m) y[i] = (beta == 0.0) ? 0.0 : beta * y[i];
+}
+
+}  // namespace
+
+uint64_t DeltaStorage::stored_bytes() const
+{
+    uint64_t b = tags.bytes() + base.bytes() + start.bytes() + val.bytes();
+    for (int t = 0; t < 4; t++) b += words[t].bytes();
+    return b;
+}
+
+void delta_build(DeltaStorage &D, int64_t m, int64_t n, const int64_t *rowptr, const int32_t *col,
+                 const double *val, hipStream_t st)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    D = DeltaStorage();
+    ABHSF_REQUIRE(m >= 1 && n >= 1 && m <= INT32_MAX && n <= INT32_MAX, ABHSF_ERR_INVALID,
+                  "matrix dimensions must be in [1, 2^31-1]");
+    ABHSF_REQUIRE(rowptr && rowptr[0] == 0, ABHSF_ERR_INVALID, "row pointers must start at 0");
+    const int64_t nnz = rowptr[m];
+    ABHSF_REQUIRE(nnz >= 0 && (uint64_t)nnz < (1ull << 32), ABHSF_ERR_UNSUPPORTED, "nnz must be in [0, 2^32)");
+    ABHSF_REQUIRE(nnz == 0 || col, ABHSF_ERR_INVALID, "null column array");
+    D.m = m;
+    D.n = n;
+    D.nnz = nnz;
+    D.K = std::max<int>(1, (int)ceil_log2_u64((uint64_t)n));
+    D.nchunk = (nnz + kDeltaChunk - 1) / kDeltaChunk;
+    std::vector<uint32_t> tags((size_t)(nnz + 15) / 16, 0u);
+    std::vector<uint8_t> w8;
+    std::vector<uint16_t> w16;
+    std::vector<uint32_t> w32;
+    std::vector<uint64_t> w64;
+    std::vector<uint64_t> base((size_t)D.nchunk);
+    std::vector<uint32_t> start((size_t)D.nchunk * 4);
+    uint64_t prev = 0;
+    int64_t e = 0;
+    for (int64_t r = 0; r < m; r++) {
+        ABHSF_REQUIRE(rowptr[r + 1] >= rowptr[r] && rowptr[r + 1] <= nnz, ABHSF_ERR_INVALID,
+                      "row pointers not monotone");
+        for (int64_t q = rowptr[r]; q < rowptr[r + 1]; q++, e++) {
+            const int32_t cc = col[q];
+            ABHSF_REQUIRE(cc >= 0 && (int64_t)cc < n, ABHSF_ERR_INVALID, "column out of range");
+            ABHSF_REQUIRE(q == rowptr[r] || cc > col[q - 1], ABHSF_ERR_INVALID,
+                          "columns not strictly increasing within a row");
+            const uint64_t key = ((uint64_t)r << D.K) + (uint64_t)cc;
+            if (e % kDeltaChunk == 0) {
+                const int64_t c = e / kDeltaChunk;
+                base[(size_t)c] = prev;
+                start[(size_t)c * 4 + 0] = (uint32_t)w8.size();
+                start[(size_t)c * 4 + 1] = (uint32_t)w16.size();
+                start[(size_t)c * 4 + 2] = (uint32_t)w32.size();
+                start[(size_t)c * 4 + 3] = (uint32_t)w64.size();
+            }
+            const uint64_t delta = key - prev;
+            uint32_t tag;
+            if (delta < (1ull << 6)) { tag = 0; w8.push_back((uint8_t)(delta << 2)); }
+            else if (delta < (1ull << 14)) { tag = 1; w16.push_back((uint16_t)((delta << 2) | 1u)); }
+            else if (delta < (1ull << 30)) { tag = 2; w32.push_back((uint32_t)((delta << 2) | 2u)); }
+            else { tag = 3; w64.push_back((delta << 2) | 3u); }
+            tags[(size_t)(e >> 4)] |= tag << (2 * (e & 15));
+            prev = key;
+        }
+    }
+    D.count[0] = w8.size();
+    D.count[1] = w16.size();
+    D.count[2] = w32.size();
+    D.count[3] = w64.size();
+    auto up = [&](DevBuf &d, const void *src, size_t bytes) {
+        d.alloc(bytes);
+        if (bytes) ABHSF_HIP(hipMemcpyAsync(d.get(), src, bytes, hipMemcpyHostToDevice, st));
+    };
+    up(D.tags, tags.data(), tags.size() * 4);
+    up(D.words[0], w8.data(), w8.size());
+    up(D.words[1], w16.data(), w16.size() * 2);
+    up(D.words[2], w32.data(), w32.size() * 4);
+    up(D.words[3], w64.data(), w64.size() * 8);
+    up(D.base, base.data(), base.size() * 8);
+    up(D.start, start.data(), start.size() * 4);
+    D.val.alloc((size_t)nnz * 8);
+    if (nnz) {
+        if (val) {
+            ABHSF_HIP(hipMemcpyAsync(D.val.get(), val, (size_t)nnz * 8, hipMemcpyHostToDevice, st));
+        } else {
+            std::vector<double> ones((size_t)nnz, 1.0);
+            ABHSF_HIP(hipMemcpyAsync(D.val.get(), ones.data(), (size_t)nnz * 8, hipMemcpyHostToDevice, st));
+            ABHSF_HIP(hipStreamSynchronize(st));
+        }
+    }
+    ABHSF_HIP(hipStreamSynchronize(st));
+    D.built = true;
+    D.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void delta_spmv(const DeltaStorage &D, double alpha, const double *x, double beta, double *y, hipStream_t st)
+{
+    ABHSF_REQUIRE(D.built, ABHSF_ERR_STATE, "matrix not built");
+    if (beta != 1.0) {
+        hipLaunchKernelGGL(k_delta_scale, dim3((unsigned)((D.m + 255) / 256)), dim3(256), 0, st, y, D.m, beta);
+        ABHSF_HIP(hipGetLastError());
+    }
+    if (D.nchunk == 0 || alpha == 0.0) return;
+    const int64_t nblk = (D.nchunk + 3) / 4;
+    ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many chunks");
+    hipLaunchKernelGGL(k_delta_spmv, dim3((unsigned)nblk), dim3(256), 0, st, D.tags.get<uint32_t>(),
+                       D.words[0].get<uint8_t>(), D.words[1].get<uint16_t>(), D.words[2].get<uint32_t>(),
+                       D.words[3].get<uint64_t>(), D.base.get<uint64_t>(), D.start.get<uint32_t>(),
+                       D.val.get<double>(), D.nnz, D.K, D.nchunk, x, y, alpha);
+    ABHSF_HIP(hipGetLastError());
+}
+
+}  // namespace abhsf

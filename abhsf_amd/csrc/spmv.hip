@@ -27,6 +27,7 @@
 // "Accurate sum and dot product", SIAM J. Sci. Comput. 26(6), 2005, Dot2).
 #include "spmv.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace abhsf {
@@ -1234,7 +1235,12 @@ void spmm(const SpmvLaunch &L, const DevMat &A, int nrhs, double alpha, const do
           double *Y, int64_t ldy, hipStream_t st)
 {
     const bool comp = L.val_bytes == 4;
-    const int rmax = max_rhs_tile(A.s, comp);
+    // column tiles of at most 8 right-hand sides (16 with dense blocks, whose
+    // tiles go through the FP64 matrix cores from R = 16): wider tiles hold a
+    // larger s x R accumulator in LDS per wave and lose more occupancy than
+    // the extra pass over the matrix costs (stencil27, s = 64: k = 32 as one
+    // tile 12.2 ms, as 2 x 16 4.3 ms each, as 4 x 8 1.5 ms each)
+    const int rmax = std::min(max_rhs_tile(A.s, comp), L.has_dense && !comp ? 16 : 8);
     for (int j0 = 0; j0 < nrhs; j0 += rmax) {
         const int nr = nrhs - j0 < rmax ? nrhs - j0 : rmax;
         int R = 1;

@@ -30,6 +30,7 @@ struct SpmvLaunch {
     int bcol_bytes;          // 2 or 4
     int val_bytes;           // 8 (f64) or 4 (f32, compensated accumulation)
     bool coo_single = false; // COO blocks hold about one entry on average
+    bool has_dense = false;  // some dense-scheme blocks (SpMM: matrix-core tiles at R >= 16)
     Balance bal;
 };
 

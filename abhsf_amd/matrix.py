@@ -247,3 +247,79 @@ class AbhsfBatch:
             self.release()
         except Exception:
             pass
+
+
+class DeltaMatrix:
+    """Delta-coded coordinate storage resident in HBM: the reference's
+    delta_matrix (spmv/unified/spmv.cpp:628-1033) with its tagged varint
+    words laid out for wavefront decode (see include/abhsf_c.h)."""
+
+    def __init__(self, stream=None):
+        self._stream = stream
+        self._h = C.c_void_p(0)
+        capi.call("abhsf_delta_create", stream.handle if isinstance(stream, hip.Stream) else stream,
+                  C.byref(self._h))
+
+    def from_elements(self, elements, props: matrix_properties) -> None:
+        """Build from (rows, cols, vals) arrays or (row, col, val) tuples; a
+        private copy is sorted row-major (the reference sorts in place)."""
+        if isinstance(elements, tuple) and len(elements) == 3 and hasattr(elements[0], "__len__"):
+            rows, cols, vals = (np.asarray(a) for a in elements)
+        else:
+            arr = np.asarray(elements, dtype=np.float64).reshape(-1, 3)
+            rows, cols, vals = arr[:, 0], arr[:, 1], arr[:, 2]
+        order = np.lexsort((cols, rows))
+        rows = rows[order].astype(np.int64)
+        cols = cols[order].astype(np.int32)
+        vals = np.ascontiguousarray(vals[order], dtype=np.float64)
+        rowptr = np.zeros(props.m + 1, dtype=np.int64)
+        np.add.at(rowptr, rows + 1, 1)
+        self.from_csr(props.m, props.n, np.cumsum(rowptr), cols, vals)
+
+    def from_csr(self, m: int, n: int, rowptr, col, val, on_device: bool = False, nnz: int = None) -> None:
+        if on_device:
+            capi.call("abhsf_delta_build_csr", self._h, m, n, col.shape[0] if nnz is None else nnz,
+                      _addr(rowptr), _addr(col), None if val is None else _addr(val), 1)
+            return
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = None if val is None else np.ascontiguousarray(val, dtype=np.float64)
+        capi.call("abhsf_delta_build_csr", self._h, m, n, int(rowptr[-1]), rowptr.ctypes.data_as(C.c_void_p),
+                  col.ctypes.data_as(C.c_void_p), None if val is None else val.ctypes.data_as(C.c_void_p), 0)
+
+    def spmv(self, x, y) -> None:
+        """y += A*x and wait (delta_matrix::spmv, spmv/unified/spmv.cpp:712-789)."""
+        capi.call("abhsf_delta_spmv", self._h, 1.0, _addr(x), 1.0, _addr(y))
+        capi.call("abhsf_delta_synchronize", self._h)
+
+    def spmv_async(self, alpha: float, x, beta: float, y) -> None:
+        capi.call("abhsf_delta_spmv", self._h, float(alpha), _addr(x), float(beta), _addr(y))
+
+    def synchronize(self) -> None:
+        capi.call("abhsf_delta_synchronize", self._h)
+
+    def stats(self) -> dict:
+        st = capi.DeltaStats()
+        capi.call("abhsf_delta_get_stats", self._h, C.byref(st))
+        return st.as_dict()
+
+    def export(self, name: str) -> np.ndarray:
+        dt = {"tags": np.uint32, "w8": np.uint8, "w16": np.uint16, "w32": np.uint32, "w64": np.uint64,
+              "base": np.uint64, "start": np.uint32, "val": np.float64}[name]
+        nbytes = C.c_size_t(0)
+        capi.call("abhsf_delta_export", self._h, name.encode(), None, 0, C.byref(nbytes))
+        out = np.empty(nbytes.value // np.dtype(dt).itemsize, dtype=dt)
+        capi.call("abhsf_delta_export", self._h, name.encode(), out.ctypes.data_as(C.c_void_p), nbytes.value,
+                  C.byref(nbytes))
+        return out
+
+    def release(self) -> None:
+        if self._h and self._h.value:
+            capi.call("abhsf_delta_destroy", self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

@@ -226,6 +226,40 @@ abhsf_status abhsf_mlp_backward(const float *theta, int32_t nfeat, const float *
 abhsf_status abhsf_mlp_train(float *theta, int32_t nfeat, const float *X, const float *y, const float *w,
                              int64_t nsamples, int64_t steps, float lr, float *loss_host, void *stream);
 
+/* ---- Delta-coded coordinate storage (spmv/unified/spmv.cpp:628-1033) ----
+ * The reference's delta_matrix family stores the deltas of linearised keys
+ * key = (row << K) + col of the row-major elements as tagged varint words
+ * ((delta << 2) | tag, 1/2/4/8 bytes for delta < 2^6 / 2^14 / 2^30 / more,
+ * :859-896) and decodes them with a serial running sum (:923-944).  Here the
+ * same words are kept in one aligned stream per width, the 2-bit tags apart,
+ * and chunks of 512 elements carry their base key and stream offsets, so a
+ * wavefront decodes 64 elements per step (ballots, scan).  Replaces
+ * delta_matrix::from_elements / spmv / release (:635-710, 712-789, 790-800). */
+typedef struct abhsf_delta abhsf_delta;
+
+typedef struct abhsf_delta_stats {
+    int64_t m, n, nnz;
+    int32_t key_bits;          /* K of key = (row << K) + col */
+    int32_t chunk;             /* elements per chunk */
+    uint64_t words[4];         /* elements coded in 1 / 2 / 4 / 8 bytes */
+    uint64_t stored_bytes;     /* all arrays as stored in HBM */
+    double build_ms;           /* host encode + upload */
+} abhsf_delta_stats;
+
+abhsf_status abhsf_delta_create(void *stream, abhsf_delta **out);
+/* CSR input as for abhsf_build_csr; the encoder runs on the host. */
+abhsf_status abhsf_delta_build_csr(abhsf_delta *d, int64_t m, int64_t n, int64_t nnz, const int64_t *rowptr,
+                                   const int32_t *col, const double *val, int32_t on_device);
+/* y = alpha*A*x + beta*y on the handle's stream; beta = 1 is the reference's
+ * accumulating delta_matrix::spmv(x, y). */
+abhsf_status abhsf_delta_spmv(abhsf_delta *d, double alpha, const double *x, double beta, double *y);
+abhsf_status abhsf_delta_get_stats(const abhsf_delta *d, abhsf_delta_stats *out);
+/* Arrays: "tags" u32, "w8" u8, "w16" u16, "w32" u32, "w64" u64, "base" u64,
+ * "start" u32[chunks][4], "val" f64.  dst NULL: size only. */
+abhsf_status abhsf_delta_export(const abhsf_delta *d, const char *name, void *dst, size_t cap, size_t *bytes);
+abhsf_status abhsf_delta_synchronize(abhsf_delta *d);
+abhsf_status abhsf_delta_destroy(abhsf_delta *d);
+
 /* Wait for all work enqueued on the handle's stream. */
 abhsf_status abhsf_synchronize(abhsf_handle *h);
 

@@ -604,3 +604,40 @@ uint64_t orc_decode(const orc_abhsf *A, int64_t *rows, int64_t *cols, double *va
 #undef EMIT
     return o;
 }
+
+
+/* ---- tagged varint delta codec (spmv/unified/spmv.cpp:859-896, 923-944) ---- */
+
+uint64_t orc_varint_encode(uint64_t nnz, const uint64_t *keys, uint8_t *out)
+{
+    uint64_t pos = 0;
+    for (uint64_t k = 1; k < nnz; k++) {
+        const uint64_t delta = keys[k] - keys[k - 1];
+        uint64_t word;
+        int bytes;
+        if (delta < (1ull << 6)) { word = delta << 2; bytes = 1; }
+        else if (delta < (1ull << 14)) { word = (delta << 2) | 1u; bytes = 2; }
+        else if (delta < (1ull << 30)) { word = (delta << 2) | 2u; bytes = 4; }
+        else { word = (delta << 2) | 3u; bytes = 8; }
+        if (out)
+            for (int b = 0; b < bytes; b++) out[pos + (uint64_t)b] = (uint8_t)(word >> (8 * b));
+        pos += (uint64_t)bytes;
+    }
+    return pos;
+}
+
+uint64_t orc_varint_decode(uint64_t nnz, uint64_t first, const uint8_t *in, uint64_t *keys)
+{
+    uint64_t pos = 0;
+    if (nnz == 0) return 0;
+    keys[0] = first;
+    for (uint64_t k = 1; k < nnz; k++) {
+        const int tag = in[pos] & 3;
+        const int bytes = 1 << tag;
+        uint64_t word = 0;
+        for (int b = 0; b < bytes; b++) word |= (uint64_t)in[pos + (uint64_t)b] << (8 * b);
+        keys[k] = keys[k - 1] + (word >> 2);
+        pos += (uint64_t)bytes;
+    }
+    return pos;
+}

@@ -111,6 +111,19 @@ uint64_t orc_decode(const orc_abhsf *A, int64_t *rows, int64_t *cols, double *va
  * (block_statistics/rows-nnz-stdev.cpp:11-30). */
 double orc_rows_stdev(uint64_t m, const uint64_t *rows_nnz);
 
+/* Tagged varint delta stream of linearised keys (spmv/unified/spmv.cpp:
+ * 859-896, one thread): keys[0] is the start value (not encoded); element
+ * k >= 1 stores delta = keys[k] - keys[k-1] little-endian as
+ * (delta << 2) | tag in 1 byte (delta < 2^6, tag 0), 2 bytes (< 2^14, tag 1),
+ * 4 bytes (< 2^30, tag 2) or 8 bytes (tag 3).  out may be NULL (size only);
+ * returns the stream length in bytes. */
+uint64_t orc_varint_encode(uint64_t nnz, const uint64_t *keys, uint8_t *out);
+
+/* Decode of the same stream with a running sum (spmv/unified/spmv.cpp:
+ * 923-944): keys[0] = first, keys[k] = keys[k-1] + delta.  Returns the bytes
+ * consumed. */
+uint64_t orc_varint_decode(uint64_t nnz, uint64_t first, const uint8_t *in, uint64_t *keys);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

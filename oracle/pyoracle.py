@@ -68,6 +68,10 @@ def lib():
         L.orc_csr_spmv.argtypes = [i64, vp, vp, vp, d, vp, d, vp]
         L.orc_decode.restype = u64
         L.orc_decode.argtypes = [C.POINTER(OrcAbhsf), vp, vp, vp, u64]
+        L.orc_varint_encode.restype = u64
+        L.orc_varint_encode.argtypes = [u64, vp, vp]
+        L.orc_varint_decode.restype = u64
+        L.orc_varint_decode.argtypes = [u64, u64, vp, vp]
         L.orc_rows_stdev.restype = d
         L.orc_rows_stdev.argtypes = [u64, vp]
         L.orc_num_threads.restype = C.c_int
@@ -119,6 +123,23 @@ def mmf_row(m, n, nnz, r, s, hist, is_binary=False, bits_single=32, mult=2):
 def rows_stdev(rows_nnz):
     a = np.ascontiguousarray(rows_nnz, dtype=np.uint64)
     return float(lib().orc_rows_stdev(a.size, _p(a)))
+
+
+def varint_encode(keys):
+    """Reference tagged varint delta stream of ascending keys (bytes after
+    the first key): spmv/unified/spmv.cpp:859-896."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    n = lib().orc_varint_encode(keys.size, _p(keys), None)
+    out = np.zeros(n, dtype=np.uint8)
+    lib().orc_varint_encode(keys.size, _p(keys), _p(out))
+    return out
+
+
+def varint_decode(stream, first, nnz):
+    stream = np.ascontiguousarray(stream, dtype=np.uint8)
+    keys = np.zeros(nnz, dtype=np.uint64)
+    used = lib().orc_varint_decode(nnz, int(first), _p(stream), _p(keys))
+    return keys, int(used)
 
 
 def csr_spmv(rowptr, col, val, x, alpha=1.0, beta=0.0, y=None):

@@ -14,6 +14,9 @@ transposed operand fails at least one):
   rows_stdev    golden value; numpy population stdev.
   build/decode  round trip; popcount(bitmap word) = z; hand-built 8x8 layouts.
   spmm          textbook CSR SpMV; brute-force dense product; closed forms.
+  varint        hand-encoded words at every width boundary of
+                spmv/unified/spmv.cpp:859-896 (2^6, 2^14, 2^30), the byte
+                order, decode(encode(k)) = k on random and 62-bit keys.
 """
 import json
 import os
@@ -318,3 +321,27 @@ def test_planted_mistake_is_caught():
     assert total(lg(s * s), s + 1) == KA["laplacian_256"]["preproc_smf_bits"]["64"]
     assert total(lg(s * s), s) != KA["laplacian_256"]["preproc_smf_bits"]["64"]
     assert total(0, s + 1) != KA["laplacian_256"]["preproc_smf_bits"]["64"]
+
+
+def test_varint_codec_by_hand():
+    """Widths and words of spmv/unified/spmv.cpp:859-896: (delta << 2) | tag,
+    little-endian, 1/2/4/8 bytes for delta < 2^6 / 2^14 / 2^30 / more."""
+    deltas = [0, 63, 64, 16383, 16384, (1 << 30) - 1, 1 << 30, (1 << 62) - 1]
+    want = [[0x00], [0xFC], [0x01, 0x01], [0xFD, 0xFF], [0x02, 0x00, 0x01, 0x00],
+            [0xFE, 0xFF, 0xFF, 0xFF], [0x03, 0, 0, 0, 1, 0, 0, 0], [0xFF] * 8]
+    for d, w in zip(deltas, want):
+        keys = np.array([7, 7 + d], dtype=np.uint64)
+        assert list(O.varint_encode(keys)) == w, d
+    # a 1-byte word is the delta shifted left by two with tag 0
+    keys = np.array([0, 5, 6, 70], dtype=np.uint64)
+    assert list(O.varint_encode(keys)) == [5 << 2, 1 << 2, ((64 << 2) | 1) & 0xFF, (64 << 2) >> 8]
+
+
+def test_varint_codec_roundtrip():
+    rng = np.random.default_rng(3)
+    for hi in (1 << 5, 1 << 13, 1 << 29, 1 << 40, 1 << 61):
+        n = int(min(2000, ((1 << 62) - 1) // hi))  # keys stay below 2^62
+        keys = np.cumsum(rng.integers(0, hi, n, dtype=np.int64)).astype(np.uint64)
+        s = O.varint_encode(keys)
+        got, used = O.varint_decode(s, keys[0], keys.size)
+        assert used == s.size and np.array_equal(got, keys)
