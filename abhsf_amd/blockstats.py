@@ -158,3 +158,51 @@ def storage_optimal_block_size(A, candidates=tuner.S_CANDIDATES):
 
 __all__ = ["histograms", "rows_cols_nnz", "rows_nnz_stdev", "mmf_row", "mmf_table", "block_opt",
            "min_per_block_size", "subset_regret", "storage_optimal_block_size", "ceil_log2"]
+
+
+# ---- gaps between consecutive linearised keys (the delta codecs' input) -------
+
+def morton_keys(rows, cols):
+    """Bit-interleaved keys: bit i of the row goes to bit 2i, bit i of the
+    column to bit 2i + 1 (spmv/orig/morton.cpp:22-30)."""
+    r = np.asarray(rows, dtype=np.uint64)
+    c = np.asarray(cols, dtype=np.uint64)
+    key = np.zeros(r.shape, dtype=np.uint64)
+    for i in range(32):
+        bit = np.uint64(1) << np.uint64(i)
+        key |= ((r & bit) << np.uint64(i)) | ((c & bit) << np.uint64(i + 1))
+    return key
+
+
+def delta_widths(A, order="morton", top=50):
+    """Gaps between consecutive keys in the given order ("morton", or "row":
+    key = (row << K) + col with K = ceil_log2(n), the delta format's key) and
+    their histogram by required bits, ceil_log2(gap) with -1 for a zero gap;
+    plus the `top` largest gaps (spmv/orig/morton.cpp:119-147, whose
+    distances array is one element short; here it has nnz - 1 entries)."""
+    r, c, _ = A.triplets()
+    if order == "morton":
+        keys = np.sort(morton_keys(r, c))
+    elif order == "row":
+        K = max(1, ceil_log2(A.n))
+        keys = (r.astype(np.uint64) << np.uint64(K)) + c.astype(np.uint64)
+    else:
+        raise ValueError(order)
+    gaps = np.diff(keys).astype(np.uint64)
+    hist = {}
+    if gaps.size:
+        zero = int(np.count_nonzero(gaps == 0))
+        if zero:
+            hist[-1] = zero
+        nz = gaps[gaps > 0]
+        # ceil_log2 of a positive gap: bit length of gap - 1
+        bits = np.zeros(nz.shape, dtype=np.int64)
+        v = nz - np.uint64(1)
+        while np.any(v):
+            bits += (v > 0)
+            v >>= np.uint64(1)
+        for b, cnt in zip(*np.unique(bits, return_counts=True)):
+            hist[int(b)] = int(cnt)
+    largest = sorted((int(g) for g in gaps), reverse=True)[:top] if gaps.size <= 1_000_000 else \
+        [int(g) for g in np.sort(gaps)[::-1][:top]]
+    return {"hist": dict(sorted(hist.items())), "largest": largest}

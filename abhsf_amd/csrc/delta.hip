@@ -10,7 +10,8 @@
 // deltas plus the chunk's base key gives every key.  Rows are
 // non-decreasing along the elements, so products are summed per row with a
 // segmented scan and one lane per row segment adds the sum to y atomically
-// (a row may continue into the next step or chunk).
+// (a row may continue into the next step or chunk).  Both scans are DPP row
+// shifts plus the row broadcasts (no LDS traffic).
 #include "delta.hpp"
 
 #include <chrono>
@@ -29,11 +30,34 @@ __device__ inline uint64_t shfl_up_u64(uint64_t v, int d)
     return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ inline uint64_t shfl_down_u64(uint64_t v, int d)
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline uint64_t dpp_u64(uint64_t v)
 {
-    const uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)v, d, kWaveD);
-    const uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(v >> 32), d, kWaveD);
-    return ((uint64_t)hi << 32) | lo;
+    return ((uint64_t)dpp_u32<CTRL, ROW_MASK>((uint32_t)(v >> 32)) << 32) | dpp_u32<CTRL, ROW_MASK>((uint32_t)v);
+}
+
+// Inclusive 64-bit prefix sum over the wave (DPP row shifts, then the row
+// broadcasts carry row totals up), as wave_incl_scan_dpp for 32 bits.
+__device__ inline uint64_t wave_incl_scan_u64(uint64_t v)
+{
+    v += dpp_u64<DPP_ROW_SHR1>(v);
+    v += dpp_u64<DPP_ROW_SHR2>(v);
+    v += dpp_u64<DPP_ROW_SHR4>(v);
+    v += dpp_u64<DPP_ROW_SHR8>(v);
+    v += dpp_u64<DPP_ROW_BCAST15, 0xa>(v);
+    v += dpp_u64<DPP_ROW_BCAST31, 0xc>(v);
+    return v;
+}
+
+// One step of a segmented inclusive sum: lanes not yet closed by a segment
+// head add the shifted partial sum, and inherit its head flag.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline void seg_step(double &v, uint32_t &f)
+{
+    const double vd = dpp_f64<CTRL, ROW_MASK>(v);
+    const uint32_t fd = dpp_u32<CTRL, ROW_MASK>(f);
+    if (!f) v += vd;
+    f |= fd;
 }
 
 __global__ __launch_bounds__(256) void k_delta_spmv(const uint32_t *__restrict__ tags,
@@ -76,30 +100,24 @@ __global__ __launch_bounds__(256) void k_delta_spmv(const uint32_t *__restrict__
         o2 += (uint32_t)__popcll(m2);
         o3 += (uint32_t)__popcll(m3);
         // keys: inclusive scan of the deltas (invalid lanes add 0)
-        uint64_t key = word >> 2;
-#pragma unroll
-        for (int d = 1; d < kWaveD; d <<= 1) {
-            const uint64_t u = shfl_up_u64(key, d);
-            if (lane >= d) key += u;
-        }
-        key += carry;
+        uint64_t key = wave_incl_scan_u64(word >> 2) + carry;
         carry = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(key >> 32), kWaveD - 1) << 32) |
                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, kWaveD - 1);
         const uint64_t row = valid ? key >> K : ~0ull;
         double p = valid ? alpha * val[e] * x[key & kmask] : 0.0;
-        // segmented inclusive sum over runs of equal rows
-        const uint64_t prev = shfl_up_u64(row, 1), next = shfl_down_u64(row, 1);
-        int f = (lane == 0 || prev != row) ? 1 : 0;
-#pragma unroll
-        for (int d = 1; d < kWaveD; d <<= 1) {
-            const double pu = __shfl_up(p, d, kWaveD);
-            const int fu = __shfl_up(f, d, kWaveD);
-            if (lane >= d && !f) {
-                p += pu;
-                f = fu;
-            }
-        }
-        if (valid && (lane == kWaveD - 1 || next != row)) atomicAdd(&y[row], p);
+        // segmented inclusive sum over runs of equal rows (head = first lane
+        // of a run); the last lane of each run adds the run's sum to y
+        const uint64_t prev = shfl_up_u64(row, 1);
+        const uint32_t head = (lane == 0 || prev != row) ? 1u : 0u;
+        const uint32_t next_head = (uint32_t)__shfl_down((int)head, 1, kWaveD);
+        uint32_t f = head;
+        seg_step<DPP_ROW_SHR1>(p, f);
+        seg_step<DPP_ROW_SHR2>(p, f);
+        seg_step<DPP_ROW_SHR4>(p, f);
+        seg_step<DPP_ROW_SHR8>(p, f);
+        seg_step<DPP_ROW_BCAST15, 0xa>(p, f);
+        seg_step<DPP_ROW_BCAST31, 0xc>(p, f);
+        if (valid && (lane == kWaveD - 1 || next_head)) atomicAdd(&y[row], p);
     }
 }
 

@@ -117,3 +117,38 @@ def test_rows_stats_match_reference(refdir):
         got = blockstats.rows_cols_nnz(A)
         assert list(got[0]) == rs
         assert abs(blockstats.rows_nnz_stdev(rs) - pyoracle.rows_stdev(rs)) < 1e-12
+
+
+def test_morton_keys_and_delta_widths_brute_force():
+    """morton.cpp:22-30 interleave and :119-147 gap histogram, restated by a
+    plain per-bit loop and checked on hand cases."""
+    from abhsf_amd import blockstats
+    # hand cases: row bits go to even positions, column bits to odd ones
+    assert list(blockstats.morton_keys([1, 0, 3, 2], [0, 1, 0, 3])) == [1, 2, 5, 14]
+    A = synth.random_sparse(70, 90, 0.08, seed=4)
+    r, c, _ = A.triplets()
+
+    def morton(a, b):
+        k = 0
+        for i in range(32):
+            k |= ((a >> i) & 1) << (2 * i) | ((b >> i) & 1) << (2 * i + 1)
+        return k
+    keys = sorted(morton(int(a), int(b)) for a, b in zip(r, c))
+    assert list(blockstats.morton_keys(r, c)[np.argsort(blockstats.morton_keys(r, c))]) == keys
+    hist = {}
+    gaps = [b - a for a, b in zip(keys, keys[1:])]
+    for g in gaps:
+        k = -1 if g == 0 else (0 if g <= 1 else (g - 1).bit_length())
+        hist[k] = hist.get(k, 0) + 1
+    got = blockstats.delta_widths(A, "morton")
+    assert got["hist"] == dict(sorted(hist.items()))
+    assert got["largest"] == sorted(gaps, reverse=True)[:50]
+    # row order: the delta format's keys; its gaps are the codec's deltas
+    from oracle import pyoracle
+    K = (A.n - 1).bit_length()
+    rk = (r.astype(np.uint64) << np.uint64(K)) + c.astype(np.uint64)
+    w = blockstats.delta_widths(A, "row")
+    assert sum(w["hist"].values()) == A.nnz - 1 and -1 not in w["hist"]
+    d = np.diff(rk)
+    assert pyoracle.varint_encode(rk).size == sum(1 if g < 64 else 2 if g < 1 << 14 else 4 if g < 1 << 30 else 8
+                                                  for g in d)
