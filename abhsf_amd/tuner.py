@@ -36,6 +36,7 @@ from . import capi, hip
 NAME = "mlp-2x64"
 S_CANDIDATES = (8, 16, 32, 64, 128, 256)
 VARIANTS = ("strip", "balanced", "panel")
+DELTA = (0, "delta")  # the delta-coded coordinate format (no block size), one more candidate
 HIDDEN = 64
 _HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(_HERE, "tuner_model.json")          # single-matrix SpMV latency
@@ -153,10 +154,33 @@ def feature_vector(g, sf, variant):
         sf["log2_s"], sf["log2_blk_per_entry"], *sf["blk_frac"], *sf["ent_frac"],
         sf["log2_bytes_per_entry"], sf["strip_cv"], sf["log2_max_strip"], sf["heavy_frac"], sf["log2_M"],
         1.0 if variant == "balanced" else 0.0, 1.0 if variant == "panel" else 0.0,
+        1.0 if variant == "delta" else 0.0,
     ], dtype=np.float64)
 
 
-NFEAT = 6 + 17  # 6 global + 17 per candidate (fractions are 4 values each)
+NFEAT = 6 + 18  # 6 global + 18 per candidate (fractions are 4 values each)
+
+
+def delta_features(rowptr, col, n):
+    """Per-candidate features of the delta format, exact from the CSR: the
+    fractions of elements in 1/2/4/8-byte words (in the scheme-fraction
+    slots), stored bytes per entry, log2 of the chunk count."""
+    nnz = int(rowptr[-1])
+    m = rowptr.size - 1
+    K = max(1, _ceil_log2(n))
+    rows = np.repeat(np.arange(m, dtype=np.uint64), np.diff(rowptr).astype(np.int64))
+    keys = (rows << np.uint64(K)) + np.asarray(col[:nnz], dtype=np.uint64)
+    d = np.diff(keys, prepend=np.uint64(0))
+    cls = [int(np.count_nonzero(d < (1 << 6))), int(np.count_nonzero((d >= (1 << 6)) & (d < (1 << 14)))),
+           int(np.count_nonzero((d >= (1 << 14)) & (d < (1 << 30)))), int(np.count_nonzero(d >= (1 << 30)))]
+    chunks = -(-nnz // 512)
+    stored = 8 * nnz + cls[0] + 2 * cls[1] + 4 * cls[2] + 8 * cls[3] + 4 * -(-nnz // 16) + 24 * chunks
+    tot = max(nnz, 1)
+    return {
+        "log2_s": 0.0, "log2_blk_per_entry": 0.0, "blk_frac": [0.0] * 4, "ent_frac": [c / tot for c in cls],
+        "log2_bytes_per_entry": math.log2(max(stored, 1) / tot), "strip_cv": 0.0, "log2_max_strip": 0.0,
+        "heavy_frac": 0.0, "log2_M": math.log2(max(chunks, 1)), "stored_bytes": stored,
+    }
 
 
 # Batched launches use the balanced variant only: inside one launch a whole
@@ -165,8 +189,10 @@ NFEAT = 6 + 17  # 6 global + 17 per candidate (fractions are 4 values each)
 BATCH_VARIANTS = ("balanced",)
 
 
-def candidates(variants=VARIANTS):
-    return [(s, v) for s in S_CANDIDATES for v in VARIANTS if v in variants]
+def candidates(variants=VARIANTS + ("delta",)):
+    """(s, variant) of the ABHSF configurations, then DELTA when allowed."""
+    out = [(s, v) for s in S_CANDIDATES for v in VARIANTS if v in variants]
+    return out + ([DELTA] if "delta" in variants else [])
 
 
 def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
@@ -182,16 +208,27 @@ def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
             hist, nblk, ns = block_hist(A.rowptr, A.col, A.m, A.n, s, stride, False)
         per_s[s] = size_features(A.rowptr, A.m, A.n, s, hist, nblk, ns)
     for s, v in candidates():
-        rows.append(feature_vector(g, per_s[s], v))
+        sf = per_s[s] if v != "delta" else delta_features(A.rowptr, A.col, A.n)
+        rows.append(feature_vector(g, sf, v))
     return np.array(rows), per_s
 
 
 # ---- exhaustive timing sweep (labels and regret judge) -------------------------
 
-def time_candidate(A, s, variant, d_rp, d_c, d_v, x, y, stream, reps=10):
-    from .matrix import AbhsfMatrix
-    M = AbhsfMatrix(block_size=s, variant=variant, stream=stream)
+def build_candidate(A, s, variant, d_rp, d_c, d_v, stream):
+    """The matrix object of one candidate (ABHSF at (s, variant), or the
+    delta format), built from a device CSR."""
+    from .matrix import AbhsfMatrix, DeltaMatrix
+    if variant == "delta":
+        M = DeltaMatrix(stream=stream)
+    else:
+        M = AbhsfMatrix(block_size=s, variant=variant, stream=stream)
     M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+    return M
+
+
+def time_candidate(A, s, variant, d_rp, d_c, d_v, x, y, stream, reps=10):
+    M = build_candidate(A, s, variant, d_rp, d_c, d_v, stream)
     for _ in range(2):
         M.spmv_async(1.0, x, 1.0, y)
     stream.synchronize()
@@ -340,7 +377,7 @@ def pick(model, Xc):
     """Candidate with the smallest predicted time among the variants the model
     was trained on (Xc: feature rows of every candidate), and the
     predictions."""
-    allowed = model.meta.get("variants", VARIANTS)
+    allowed = model.meta.get("variants", VARIANTS + ("delta",))
     pred = model.predict(Xc)
     cands = candidates()
     ok = [i for i, (_, v) in enumerate(cands) if v in allowed]

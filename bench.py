@@ -107,8 +107,9 @@ def csr_spmv_numpy(A, x):
 
 
 def measure_candidates(A, cands, args, hip, stream, check=True):
-    """Build and time each (s, variant); returns per-candidate results."""
-    from abhsf_amd import AbhsfMatrix
+    """Build and time each (s, variant) -- (0, "delta") is the delta-coded
+    format; returns per-candidate results."""
+    from abhsf_amd import tuner
     x = np.random.default_rng(0).uniform(-1.0, 1.0, A.n)
     dx = hip.DeviceArray.from_numpy(x)
     dy = hip.DeviceArray.zeros(A.m)
@@ -118,9 +119,11 @@ def measure_candidates(A, cands, args, hip, stream, check=True):
         ref = csr_spmv_numpy(A, x)
     res = {}
     for s, v in cands:
-        M = AbhsfMatrix(block_size=s, variant=v, stream=stream)
-        M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+        M = tuner.build_candidate(A, s, v, d_rp, d_c, d_v, stream)
         st = M.stats()
+        if v == "delta":  # same keys as the ABHSF statistics
+            st = {"stored_bytes": st["stored_bytes"], "index_bytes": st["stored_bytes"] - 8 * A.nnz,
+                  "blocks": {"words_1_2_4_8": list(st["words"].values())}, "build_ms": st["build_ms"]}
         err = None
         if ref is not None:
             dy.fill_zero(stream.handle)
@@ -155,7 +158,7 @@ def tuned_pick(A, allowed_s, hip):
     Xc, _ = tuner.matrix_features(A)
     pred = model.predict(Xc)
     cands = tuner.candidates()
-    ok = [i for i, (s, _) in enumerate(cands) if s in allowed_s]
+    ok = [i for i, c in enumerate(cands) if c[0] in allowed_s or c == tuner.DELTA]
     best = min(ok, key=lambda i: pred[i])
     return cands[best], tuner.NAME
 
@@ -289,7 +292,7 @@ def run_rmat(args, hip):
         cands = [pick]
     else:
         pick, tname = tuned_pick(A, s_allowed, hip)
-        cands = [c for c in tuner.candidates() if c[0] in s_allowed]
+        cands = [c for c in tuner.candidates() if c[0] in s_allowed or c == tuner.DELTA]
         if args.s is not None:  # a restricted run: the balanced variant only, plus the pick
             cands = [c for c in cands if c[1] == "balanced" or c == pick]
     res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
@@ -442,7 +445,7 @@ def main():
     ap.add_argument("--workload", default="block_mix", choices=["block_mix", "rmat", "spmm", "batch"])
     ap.add_argument("--s", default=None, help="block sizes to measure (block_mix default 16,32,64,128; "
                     "rmat default all tuner candidates)")
-    ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel"],
+    ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel", "delta"],
                     help="block_mix / rmat: measure only (first --s, VARIANT), no tuner (profiler passes)")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--spmm-s", type=int, default=64)
