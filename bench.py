@@ -180,7 +180,8 @@ def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
     """Per-dispatch HBM traffic of the SpMV kernels from rocprofv3 --pmc CSVs.
 
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are averaged over the
-    k_spmv_strip dispatches (plus k_spmv_fixup when present) and scaled by what
+    k_spmv_strip (or k_delta_spmv) dispatches (plus k_spmv_fixup when
+    present) and scaled by what
     the same counter reads for k_stream_copy, the calibration copy of known
     size (2 GiB read + 2 GiB written per dispatch, calib_copy_gbs above).  With
     sizes_csv (a pass over TCC_EA0_RDREQ_{32B,64B,128B}_sum and
@@ -193,7 +194,7 @@ def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
         acc = defaultdict(lambda: defaultdict(list))
         for row in csv.DictReader(open(path)):
             nm = row["Kernel_Name"]
-            kind = ("spmv" if "k_spmv_strip" in nm else "fixup" if "k_spmv_fixup" in nm
+            kind = ("spmv" if ("k_spmv_strip" in nm or "k_delta_spmv" in nm) else "fixup" if "k_spmv_fixup" in nm
                     else "copy" if "k_stream_copy" in nm else None)
             if kind:
                 acc[kind][row["Counter_Name"]].append(float(row["Counter_Value"]))
