@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libabhsf.so")
 OK, ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_UNSUPPORTED = range(6)
 POLICY_PREPROC, POLICY_MSMF = 0, 1
 F64, F32 = 0, 1
-VARIANT_AUTO, VARIANT_STRIP, VARIANT_BALANCED, VARIANT_PANEL = 0, 1, 2, 3
+VARIANT_AUTO, VARIANT_STRIP, VARIANT_BALANCED, VARIANT_PANEL, VARIANT_TILED = 0, 1, 2, 3, 4
 GENERAL, SYMMETRIC, SKEW_SYMMETRIC = 0, 1, 2
 SCHEMES = ("coo", "csr", "bitmap", "dense")
 

@@ -253,6 +253,70 @@ __global__ void k_emit(const uint32_t *__restrict__ flag, const uint32_t *__rest
     }
 }
 
+// Tiled COO order, step 1: sort key of every COO block b = (group of its
+// strip, block column, strip within the group).  Groups are consecutive strip
+// ranges numbered in strip order (a tile, or one strip), so sorting by the
+// key reorders blocks only inside tiles.  The strip of b is the last I with
+// sptr[I] <= b (empty strips share their start with the next strip).
+__global__ void k_tile_keys(const uint32_t *__restrict__ sptr, int64_t M, uint64_t nb, const void *__restrict__ bcol,
+                            int bcol_bytes, const uint32_t *__restrict__ grp, const uint32_t *__restrict__ tpos,
+                            int bits_bc, int bits_t, uint64_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    const uint64_t b = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    int64_t lo = 0, hi = M - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if ((uint64_t)sptr[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const uint64_t J = bcol_bytes == 2 ? static_cast<const uint16_t *>(bcol)[b] : static_cast<const uint32_t *>(bcol)[b];
+    keys[b] = ((uint64_t)grp[lo] << (bits_bc + bits_t)) | (J << bits_t) | (uint64_t)tpos[lo];
+    vals[b] = (uint32_t)b;
+}
+
+// Step 2: new block i takes old block perm[i]: block column copied, z1 gets
+// the strip within the tile in its bits [zb, 16); znew[i] = entry count.
+__global__ void k_tile_gather(uint64_t nb, const uint32_t *__restrict__ perm, const uint64_t *__restrict__ keys,
+                              int bits_t, int zb, const void *__restrict__ bc_old, void *__restrict__ bc_new,
+                              int bcol_bytes, const uint16_t *__restrict__ z1_old, uint16_t *__restrict__ z1_new,
+                              uint32_t *__restrict__ znew)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (i >= nb) return;
+    const uint32_t b = perm[i];
+    const uint32_t t = (uint32_t)(keys[i] & ((1ull << bits_t) - 1ull));
+    if (bcol_bytes == 2) static_cast<uint16_t *>(bc_new)[i] = static_cast<const uint16_t *>(bc_old)[b];
+    else static_cast<uint32_t *>(bc_new)[i] = static_cast<const uint32_t *>(bc_old)[b];
+    const uint32_t z1 = z1_old[b];
+    z1_new[i] = (uint16_t)(z1 | (t << zb));
+    znew[i] = z1 + 1u;
+}
+
+__global__ void k_z_count(uint64_t nb, const uint16_t *__restrict__ z1, uint32_t *__restrict__ z)
+{
+    const uint64_t b = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (b < nb) z[b] = (uint32_t)z1[b] + 1u;
+}
+
+// Step 3: the entries (values and in-block indices) of new block i move from
+// the old block's offset to the new one.
+__global__ void k_tile_move(uint64_t nb, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off_old,
+                            const uint32_t *__restrict__ off_new, const uint32_t *__restrict__ znew,
+                            const void *__restrict__ v_old, void *__restrict__ v_new, int val_bytes,
+                            const void *__restrict__ i_old, void *__restrict__ i_new, int idx_bytes)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (i >= nb) return;
+    const uint64_t src = off_old[perm[i]], dst = off_new[i];
+    const uint32_t z = znew[i];
+    for (uint32_t j = 0; j < z; j++) {
+        if (val_bytes == 8) static_cast<double *>(v_new)[dst + j] = static_cast<const double *>(v_old)[src + j];
+        else static_cast<float *>(v_new)[dst + j] = static_cast<const float *>(v_old)[src + j];
+        if (idx_bytes == 1) static_cast<uint8_t *>(i_new)[dst + j] = static_cast<const uint8_t *>(i_old)[src + j];
+        else static_cast<uint16_t *>(i_new)[dst + j] = static_cast<const uint16_t *>(i_old)[src + j];
+    }
+}
+
 template <typename T>
 T read_dev(const T *p, hipStream_t st)
 {
@@ -290,6 +354,7 @@ DevMat Storage::view() const
     DevMat A;
     A.m = m; A.n = n; A.M = M; A.N = N;
     A.s = s; A.k = k; A.W = W;
+    A.zb = coo_zb;
     for (int x = 0; x < 4; x++) {
         A.sptr[x] = sptr[x].get<uint32_t>();
         A.eptr[x] = eptr[x].get<uint32_t>();
@@ -321,20 +386,71 @@ SpmvLaunch Storage::launch() const
     L.bal.n_units = n_units;
     L.bal.n_heavy = n_heavy;
     L.bal.n_slots = n_slots;
+    L.bal.tiled = tiled;
+    L.bal.coo_rng = bal_coo_rng.get<uint32_t>();
     return L;
 }
 
-void Storage::build_balance(int64_t unit, hipStream_t st, int panels)
+// Reorder the COO group of S by the key (rank of the block's strip, block
+// column, position of the strip in its tile): grp[I] ranks strip I's group
+// (a tile or a single strip; ranks follow the groups' first strips), tpos[I]
+// is its position in the tile (0 outside tiles).  z1 gains that position in
+// its bits [coo_zb, 16).
+static void tile_coo_order(Storage &S, const std::vector<uint32_t> &grp, const std::vector<uint32_t> &tpos,
+                           uint64_t nrank, int64_t tcap, hipStream_t st)
 {
+    const int64_t M = S.M;
+    const uint64_t nb = S.nb[COO];
+    const uint64_t g = nrank - 1;
+    const int bits_bc = bits_for((uint64_t)std::max<int64_t>(S.N - 1, 1));
+    const int bits_t = bits_for((uint64_t)std::max<int64_t>(tcap - 1, 1));
+    const int key_bits = bits_for(g) + bits_bc + bits_t;
+    ABHSF_REQUIRE(key_bits <= 64, ABHSF_ERR_UNSUPPORTED, "tile sort key exceeds 64 bits");
+    DevBuf dgrp((size_t)M * 4), dtpos((size_t)M * 4), keys(nb * 8), perm(nb * 4);
+    ABHSF_HIP(hipMemcpyAsync(dgrp.get(), grp.data(), dgrp.bytes(), hipMemcpyHostToDevice, st));
+    ABHSF_HIP(hipMemcpyAsync(dtpos.get(), tpos.data(), dtpos.bytes(), hipMemcpyHostToDevice, st));
+    k_tile_keys<<<grid_for(nb), kT, 0, st>>>(S.sptr[COO].get<uint32_t>(), M, nb, S.bcol[COO].get(), S.bcol_bytes,
+                                             dgrp.get<uint32_t>(), dtpos.get<uint32_t>(), bits_bc, bits_t,
+                                             keys.get<uint64_t>(), perm.get<uint32_t>());
+    ABHSF_HIP(hipGetLastError());
+    radix_sort_pairs(keys.get<uint64_t>(), perm.get<uint32_t>(), nb, key_bits, st);
+    DevBuf bc_new(S.bcol[COO].bytes()), z1_new(S.z1[COO].bytes()), znew(nb * 4), off_old(nb * 4), off_new(nb * 4);
+    k_z_count<<<grid_for(nb), kT, 0, st>>>(nb, S.z1[COO].get<uint16_t>(), off_old.get<uint32_t>());
+    ABHSF_HIP(hipGetLastError());
+    exclusive_scan_u32(off_old.get<uint32_t>(), off_old.get<uint32_t>(), nb, nullptr, st);
+    k_tile_gather<<<grid_for(nb), kT, 0, st>>>(nb, perm.get<uint32_t>(), keys.get<uint64_t>(), bits_t, S.coo_zb,
+                                               S.bcol[COO].get(), bc_new.get(), S.bcol_bytes,
+                                               S.z1[COO].get<uint16_t>(), z1_new.get<uint16_t>(),
+                                               znew.get<uint32_t>());
+    ABHSF_HIP(hipGetLastError());
+    exclusive_scan_u32(znew.get<uint32_t>(), off_new.get<uint32_t>(), nb, nullptr, st);
+    DevBuf v_new(S.val[COO].bytes()), i_new(S.coo_idx.bytes());
+    k_tile_move<<<grid_for(nb), kT, 0, st>>>(nb, perm.get<uint32_t>(), off_old.get<uint32_t>(),
+                                             off_new.get<uint32_t>(), znew.get<uint32_t>(), S.val[COO].get(),
+                                             v_new.get(), S.val_bytes, S.coo_idx.get(), i_new.get(),
+                                             S.coo_idx_bytes);
+    ABHSF_HIP(hipGetLastError());
+    ABHSF_HIP(hipStreamSynchronize(st));
+    S.bcol[COO] = std::move(bc_new);
+    S.z1[COO] = std::move(z1_new);
+    S.val[COO] = std::move(v_new);
+    S.coo_idx = std::move(i_new);
+}
+
+void Storage::build_balance(int64_t unit, hipStream_t st, int panels, int tile_rows)
+{
+    ABHSF_REQUIRE(!tiled, ABHSF_ERR_STATE, "the COO blocks are already in tile order");
     bal_light.release();
     bal_pack.release();
     bal_units.release();
     bal_heavy.release();
     bal_part.release();
     bal_order.release();
+    bal_coo_rng.release();
     n_light = n_units = n_heavy = n_slots = 0;
     if (unit <= 0 || M == 0) return;
     ABHSF_REQUIRE(panels == 1 || panels == kXcdCount, ABHSF_ERR_INVALID, "column panels must match the XCD count");
+    ABHSF_REQUIRE(tile_rows <= 0 || panels == 1, ABHSF_ERR_INVALID, "tiles and column panels do not combine");
     std::vector<uint32_t> ep[4], sp[4];
     for (int x = 0; x < 4; x++) {
         ep[x].resize((size_t)M + 1);
@@ -351,7 +467,7 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels)
         for (int x = 0; x < 4; x++) w += (int64_t)ep[x][I + 1] - ep[x][I];
         return w;
     };
-    if (panels == 1) {  // nothing to split or pack: one item per strip, no tables
+    if (panels == 1 && tile_rows <= 0) {  // nothing to split or pack: one item per strip, no tables
         bool any = false;
         for (int64_t I = 0; I < M && !any; I++) {
             const int64_t w = strip_work(I);
@@ -476,7 +592,102 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels)
     }
     std::vector<uint32_t> pack;
     std::vector<uint8_t> item_panel;
-    {
+    // tiles: the strip within a tile goes to the COO z1 bits above the
+    // largest z - 1, and the sort key (group, block column, strip) must fit
+    // 64 bits; otherwise every tile holds one strip
+    int zb = 16;
+    int64_t tcap = 1;
+    if (tile_rows > 0) {
+        uint32_t zmax = 0;
+        for (uint64_t b = 0; b < nb[COO]; b++) zmax = std::max<uint32_t>(zmax, zz[COO][b]);
+        zb = bits_for(zmax);
+        tcap = std::min<int64_t>(tile_rows / s, zb < 16 ? (int64_t)1 << (16 - zb) : 1);
+        if (bits_for((uint64_t)M) + bits_for((uint64_t)N) + bits_for((uint64_t)std::max<int64_t>(tcap - 1, 1)) > 64)
+            tcap = 1;
+    }
+    std::vector<uint32_t> grank, gtpos, coo_rng;
+    uint64_t nrank = 0;
+    if (tile_rows > 0) {
+        // items: tiles of light strips whose entries are all COO (or none), up
+        // to tcap strips and kTileWork entries, gathered in strip order past
+        // strips with other groups (each of those stays an item of its own)
+        std::vector<std::vector<uint32_t>> items;
+        int64_t open = -1, open_work = 0;
+        for (size_t i = 0; i < light.size(); i++) {
+            const uint32_t I = light[i];
+            const int64_t cw = (int64_t)ep[COO][I + 1] - ep[COO][I];
+            const bool pure = tcap > 1 && light_work[i] == cw;
+            if (pure && open >= 0 && (int64_t)items[(size_t)open].size() < tcap && open_work + cw <= kTileWork) {
+                items[(size_t)open].push_back(I);
+                open_work += cw;
+                continue;
+            }
+            items.push_back({I});
+            if (pure) {
+                open = (int64_t)items.size() - 1;
+                open_work = cw;
+            }
+        }
+        // tiles first: they start together and sweep x in step (their COO
+        // streams run in block-column order), the single strips fill the
+        // remaining slots and follow
+        std::stable_partition(items.begin(), items.end(), [](const std::vector<uint32_t> &it) { return it.size() > 1; });
+        std::vector<uint32_t> l2;
+        l2.reserve(light.size());
+        for (auto &it : items) {
+            pack.push_back((uint32_t)l2.size());
+            l2.insert(l2.end(), it.begin(), it.end());
+        }
+        pack.push_back((uint32_t)l2.size());
+        light.swap(l2);
+        // new COO block order: groups (an item, or a heavy strip) ranked by
+        // their first strip; inside a group by (block column, strip)
+        std::vector<int64_t> item_of((size_t)M, -1), rank_of_item(items.size(), -1), strip_rank((size_t)M);
+        grank.assign((size_t)M, 0u);
+        gtpos.assign((size_t)M, 0u);
+        for (size_t q = 0; q + 1 < pack.size(); q++)
+            for (uint32_t p = pack[q]; p < pack[q + 1]; p++) {
+                item_of[light[p]] = (int64_t)q;
+                gtpos[light[p]] = p - pack[q];
+            }
+        std::vector<uint64_t> gb, ge;  // blocks and values of every rank
+        for (int64_t I = 0; I < M; I++) {
+            const int64_t it = item_of[(size_t)I];
+            int64_t r = it >= 0 ? rank_of_item[(size_t)it] : -1;
+            if (r < 0) {
+                r = (int64_t)gb.size();
+                gb.push_back(0);
+                ge.push_back(0);
+                if (it >= 0) rank_of_item[(size_t)it] = r;
+            }
+            strip_rank[(size_t)I] = r;
+            grank[(size_t)I] = (uint32_t)r;
+            gb[(size_t)r] += sp[COO][I + 1] - sp[COO][I];
+            ge[(size_t)r] += ep[COO][I + 1] - ep[COO][I];
+        }
+        nrank = gb.size();
+        std::vector<uint64_t> bs(nrank + 1, 0), es(nrank + 1, 0);
+        for (size_t r = 0; r < nrank; r++) {
+            bs[r + 1] = bs[r] + gb[r];
+            es[r + 1] = es[r] + ge[r];
+        }
+        coo_rng.resize(items.size() * 3);
+        for (size_t q = 0; q < items.size(); q++) {
+            const size_t r = (size_t)rank_of_item[q];
+            coo_rng[3 * q] = (uint32_t)bs[r];
+            coo_rng[3 * q + 1] = (uint32_t)bs[r + 1];
+            coo_rng[3 * q + 2] = (uint32_t)es[r];
+        }
+        // the COO units of a heavy strip move with the strip
+        for (size_t u = 0; u < units.size() / 4; u++) {
+            if ((units[4 * u] & 3u) != (uint32_t)COO) continue;
+            const uint32_t I = units[4 * u] >> 2;
+            const size_t r = (size_t)strip_rank[I];
+            units[4 * u + 1] = (uint32_t)(units[4 * u + 1] - sp[COO][I] + bs[r]);
+            units[4 * u + 2] = (uint32_t)(units[4 * u + 2] - sp[COO][I] + bs[r]);
+            units[4 * u + 3] = (uint32_t)(units[4 * u + 3] - ep[COO][I] + es[r]);
+        }
+    } else {
         int64_t iw = 0;
         bool tiny_item = false;  // the open item packs tiny strips
         for (size_t i = 0; i < light.size(); i++) {
@@ -526,6 +737,18 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels)
     if (!order.empty())
         ABHSF_HIP(hipMemcpyAsync(bal_order.get(), order.data(), order.size() * 4, hipMemcpyHostToDevice, st));
     ABHSF_HIP(hipStreamSynchronize(st));
+    if (tile_rows > 0) {
+        bool multi = false;
+        for (size_t i = 0; i + 1 < pack.size() && !multi; i++) multi = pack[i + 1] - pack[i] > 1;
+        coo_zb = zb;
+        if (multi && nb[COO] > 0) tile_coo_order(*this, grank, gtpos, nrank, tcap, st);
+        bal_coo_rng.alloc(coo_rng.size() * 4);
+        if (!coo_rng.empty())
+            ABHSF_HIP(hipMemcpyAsync(bal_coo_rng.get(), coo_rng.data(), coo_rng.size() * 4, hipMemcpyHostToDevice,
+                                     st));
+        ABHSF_HIP(hipStreamSynchronize(st));
+        tiled = true;
+    }
 }
 
 uint64_t Storage::value_bytes() const
@@ -539,7 +762,7 @@ uint64_t Storage::stored_bytes() const
 {
     uint64_t b = value_bytes() + coo_idx.bytes() + csr_rowptr.bytes() + csr_col.bytes() + bm_words.bytes() +
                  bal_light.bytes() + bal_pack.bytes() + bal_units.bytes() + bal_heavy.bytes() +
-                 bal_order.bytes();
+                 bal_order.bytes() + bal_coo_rng.bytes();
     for (int x = 0; x < 4; x++) b += sptr[x].bytes() + eptr[x].bytes() + bcol[x].bytes() + z1[x].bytes();
     return b;
 }

@@ -100,14 +100,18 @@ int64_t balance_unit(const abhsf::Storage &S) { return std::max<int64_t>(2048, 8
 
 void finish_build(abhsf_handle *h)
 {
+    const int tile_rows =
+        h->opts.variant != ABHSF_VARIANT_TILED ? 0 : (h->S.val_bytes == 4 ? abhsf::kTileRowsComp : abhsf::kTileRows);
     if (h->opts.variant != ABHSF_VARIANT_STRIP)
         h->S.build_balance(balance_unit(h->S), h->stream,
-                           h->opts.variant == ABHSF_VARIANT_PANEL ? abhsf::kXcdCount : 1);
+                           h->opts.variant == ABHSF_VARIANT_PANEL ? abhsf::kXcdCount : 1, tile_rows);
 }
 
 void check_symmetric_shape(const abhsf_handle *h, int64_t m, int64_t n)
 {
     if (h->symmetry == ABHSF_GENERAL) return;
+    ABHSF_REQUIRE(h->opts.variant != ABHSF_VARIANT_TILED, ABHSF_ERR_UNSUPPORTED,
+                  "symmetric storage needs strip-ordered COO blocks (not the tiled variant)");
     ABHSF_REQUIRE(m == n, ABHSF_ERR_INVALID, "symmetric storage needs a square matrix");
     ABHSF_REQUIRE(h->opts.precision == ABHSF_F64, ABHSF_ERR_UNSUPPORTED, "symmetric storage needs FP64 values");
 }
@@ -156,7 +160,7 @@ abhsf_status abhsf_create(const abhsf_options *opts, abhsf_handle **out)
                       "unknown policy");
         ABHSF_REQUIRE(opts->precision == ABHSF_F64 || opts->precision == ABHSF_F32, ABHSF_ERR_INVALID,
                       "unknown precision");
-        ABHSF_REQUIRE(opts->variant >= ABHSF_VARIANT_AUTO && opts->variant <= ABHSF_VARIANT_PANEL, ABHSF_ERR_INVALID,
+        ABHSF_REQUIRE(opts->variant >= ABHSF_VARIANT_AUTO && opts->variant <= ABHSF_VARIANT_TILED, ABHSF_ERR_INVALID,
                       "unknown variant");
         *out = nullptr;
         abhsf_handle *h = new abhsf_handle();
@@ -286,6 +290,8 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
         ABHSF_REQUIRE(h->S.built, ABHSF_ERR_STATE, "matrix not built");
         ABHSF_REQUIRE(nrhs >= 1 && nrhs <= 64, ABHSF_ERR_UNSUPPORTED, "nrhs must be in [1, 64]");
         ABHSF_REQUIRE(ldx >= nrhs && ldy >= nrhs, ABHSF_ERR_INVALID, "leading dimension smaller than nrhs");
+        ABHSF_REQUIRE(!h->S.tiled || (nrhs == 1 && ldx == 1 && ldy == 1), ABHSF_ERR_UNSUPPORTED,
+                      "a tiled matrix supports SpMV only (one contiguous right-hand side)");
         abhsf::spmm(h->S.launch(), h->S.view(), nrhs, alpha, X, ldx, beta, Y, ldy, h->stream);
         if (h->built_symmetry != ABHSF_GENERAL && alpha != 0.0)  // the mirrored triangle
             abhsf::spmv_transposed(h->S.launch(), h->S.view(), nrhs,
@@ -314,6 +320,8 @@ abhsf_status abhsf_export(const abhsf_handle *h, const char *name, void *dst, si
     return guarded([&] {
         ABHSF_REQUIRE(h && name, ABHSF_ERR_INVALID, "null argument");
         ABHSF_REQUIRE(h->S.built, ABHSF_ERR_STATE, "matrix not built");
+        ABHSF_REQUIRE(!h->S.tiled, ABHSF_ERR_UNSUPPORTED,
+                      "a tiled matrix cannot be exported (its COO blocks are in tile order)");
         const abhsf::Storage &S = h->S;
         static const char *pre[4] = {"coo_", "csr_", "bm_", "dn_"};
         const abhsf::DevBuf *buf = nullptr;

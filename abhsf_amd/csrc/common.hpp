@@ -171,6 +171,7 @@ __device__ inline void two_sum(double a, double b, double &s, double &e)
 struct DevMat {
     int64_t m, n, M, N;
     int32_t s, k, W;  // W = u64 words per bitmap block
+    int32_t zb;       // COO z1: z - 1 in the low zb bits, strip within the tile above (16: untiled)
     const uint32_t *sptr[4];
     const uint32_t *eptr[4];
     const void *bcol[4];

@@ -180,14 +180,18 @@ __device__ inline void csr_rows_r1(const DevMat &A, uint32_t b0, uint32_t b1, ui
 // entries of its own block (independent loads, no lookup); phase B, only when
 // some block of the chunk holds more than ZS entries: the remaining entries
 // are spread one per lane and located by a binary search over the chunk's
-// prefix sums in LDS.  The next chunk's block metadata is loaded ahead.
-template <typename VT, typename BT, typename IT, bool COMP, int ZS, int CH>
+// prefix sums in LDS.  A block's z1 holds z - 1 in its low A.zb bits and, in
+// a tile, the block's strip t within the tile above them: t << 2k is added to
+// its in-block indices, so (index >> k) = (t << k) + lrow is the tile row and
+// the column bits are unchanged.
+template <typename VT, typename BT, typename IT, bool COMP, int ZS, int CH, bool TILE = false>
 __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_t e, const double *__restrict__ x,
-                              const Acc &acc, uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, int lane)
+                              const Acc &acc, uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, uint32_t *s_to,
+                              int lane)
 {
     if (b0 >= b1) return;
-    const int s = A.s, k = A.k;
-    const uint32_t smask = (uint32_t)s - 1;
+    const int s = A.s, k = A.k, zb = TILE ? A.zb : 16;
+    const uint32_t smask = (uint32_t)s - 1, zmask = (1u << zb) - 1u;
     const VT *__restrict__ val = static_cast<const VT *>(A.val[COO]);
     const IT *__restrict__ idx = static_cast<const IT *>(A.coo_idx);
     const uint16_t *__restrict__ z1 = A.z1[COO];
@@ -196,18 +200,23 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
     // any of them is consumed, so several memory round trips overlap.
 #pragma unroll 1
     for (uint32_t cb = b0; cb < b1; cb += (uint32_t)(kWave * CH)) {
-        uint32_t z[CH], bc[CH], st[CH];
+        // zt = z | (strip within the tile << 17)  (z <= 2^16)
+        uint32_t zt[CH], bc[CH], st[CH];
 #pragma unroll
         for (int c = 0; c < CH; c++) {
             const uint32_t jb = cb + (uint32_t)(c * kWave) + (uint32_t)lane;
             const bool ok = jb < b1;
-            z[c] = ok ? (uint32_t)z1[jb] + 1u : 0u;
+            const uint32_t r = ok ? (uint32_t)z1[jb] : 0u;
+            if constexpr (TILE) zt[c] = ok ? ((r & zmask) + 1u) | ((r >> zb) << 17) : 0u;
+            else zt[c] = ok ? r + 1u : 0u;
             bc[c] = ok ? (uint32_t)bcol[jb] : 0u;
         }
+        auto zof = [&](int c) { return TILE ? zt[c] & 0x1FFFFu : zt[c]; };
+        auto tzof = [&](int c) { return TILE ? (zt[c] >> 17) << (2 * k) : 0u; };
 #pragma unroll
         for (int c = 0; c < CH; c++) {
-            const uint32_t incl = wave_incl_scan_dpp(z[c]);
-            st[c] = e + incl - z[c];
+            const uint32_t incl = wave_incl_scan_dpp(zof(c));
+            st[c] = e + incl - zof(c);
             e += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
         }
         uint32_t id[CH][ZS];
@@ -216,20 +225,20 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
         for (int c = 0; c < CH; c++)
 #pragma unroll
             for (int i = 0; i < ZS; i++) {
-                const bool h = (uint32_t)i < z[c];
-                id[c][i] = h ? (uint32_t)idx[st[c] + i] : 0u;
+                const bool h = (uint32_t)i < zof(c);
+                id[c][i] = h ? (uint32_t)idx[st[c] + i] + tzof(c) : 0u;
                 v[c][i] = h ? (double)val[st[c] + i] : 0.0;
             }
 #pragma unroll
         for (int c = 0; c < CH; c++)
 #pragma unroll
             for (int i = 0; i < ZS; i++)
-                xv[c][i] = ((uint32_t)i < z[c]) ? x[(int64_t)bc[c] * s + (id[c][i] & smask)] : 0.0;
+                xv[c][i] = ((uint32_t)i < zof(c)) ? x[(int64_t)bc[c] * s + (id[c][i] & smask)] : 0.0;
 #pragma unroll
         for (int c = 0; c < CH; c++)
 #pragma unroll
             for (int i = 0; i < ZS; i++)
-                if ((uint32_t)i < z[c]) {
+                if ((uint32_t)i < zof(c)) {
                     double pe;
                     const double p = prod<COMP>(v[c][i], xv[c][i], pe);
                     acc_atomic<COMP>(acc, id[c][i] >> k, p, pe);
@@ -237,13 +246,14 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
         // phase B: entries beyond the first ZS of a block, one per lane
 #pragma unroll 1
         for (int c = 0; c < CH; c++) {
-            const uint32_t rem = z[c] > (uint32_t)ZS ? z[c] - (uint32_t)ZS : 0u;
+            const uint32_t rem = zof(c) > (uint32_t)ZS ? zof(c) - (uint32_t)ZS : 0u;
             if (__ballot(rem != 0) == 0) continue;
             const uint32_t rincl = wave_incl_scan_dpp(rem);
             const uint32_t RT = (uint32_t)__builtin_amdgcn_readlane((int)rincl, kWave - 1);
             s_end[lane] = rincl;
             s_bc[lane] = bc[c];
             s_st[lane] = st[c] + (uint32_t)ZS;
+            if constexpr (TILE) s_to[lane] = tzof(c);
             wave_sync();
             const uint32_t cs = cb + (uint32_t)(c * kWave);
             const int nbk = (int)min<uint32_t>(kWave, b1 - cs);
@@ -251,7 +261,7 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
             // loads, then the updates
 #pragma unroll 1
             for (uint32_t q = 0; q < RT; q += 2 * kWave) {
-                uint32_t pos[2], xb[2], ide[2];
+                uint32_t pos[2], xb[2], ide[2], rb[2];
                 double ve[2], xe[2];
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
@@ -263,11 +273,12 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
                     }
                     pos[u] = s_st[lo] + t - (lo ? s_end[lo - 1] : 0u);
                     xb[u] = s_bc[lo];
+                    rb[u] = TILE ? s_to[lo] : 0u;
                 }
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
                     const bool h = q + (uint32_t)(u * kWave + lane) < RT;
-                    ide[u] = h ? (uint32_t)idx[pos[u]] : 0u;
+                    ide[u] = h ? (uint32_t)idx[pos[u]] + rb[u] : 0u;
                     ve[u] = h ? (double)val[pos[u]] : 0.0;
                 }
 #pragma unroll
@@ -424,7 +435,7 @@ __device__ inline int64_t xcd_block_id()
 template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS = 2, int CH = 4>
 __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item, const double *__restrict__ x,
                                  double *__restrict__ y, double alpha, double beta, double *acc_buf,
-                                 uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, int lane)
+                                 uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, uint32_t *s_to, int lane)
 {
     const int s = A.s;
     Acc acc;
@@ -442,6 +453,50 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
             p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item]);
             p1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item + 1]);
         }
+        if (B.tiled) {
+            // the item's strips light[p0 .. p1): one pass over their COO
+            // blocks (rows (position << k) + lrow), then each strip's CSR /
+            // bitmap / dense groups, then all rows
+            const uint32_t *R = B.coo_rng + (uint64_t)item * 3;
+            const uint32_t cb0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)R[0]);
+            const uint32_t cb1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)R[1]);
+            const uint32_t ce0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)R[2]);
+            const int T = (int)(p1 - p0);
+            const int rows = T * s;
+            for (int i = s + lane; i < rows; i += kWave) {
+                acc.hi[i] = 0.0;
+                if constexpr (COMP) acc.lo[i] = 0.0;
+            }
+            wave_sync();
+            if (T > 1)
+                coo_r1<VT, BT, IT, COMP, ZS, CH, true>(A, cb0, cb1, ce0, x, acc, s_end, s_bc, s_st, s_to, lane);
+            else
+                coo_r1<VT, BT, IT, COMP, ZS, CH>(A, cb0, cb1, ce0, x, acc, s_end, s_bc, s_st, s_to, lane);
+#pragma unroll 1
+            for (int t = 0; t < T; t++) {
+                const int64_t I = (int64_t)__builtin_amdgcn_readfirstlane((int)B.light[p0 + (uint32_t)t]);
+                Acc at;
+                at.hi = acc.hi + t * s;
+                at.lo = COMP ? acc.lo + t * s : nullptr;
+                csr_rows_r1<VT, BT, RT, COMP>(A, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, at, s_st,
+                                              s_bc, lane);
+                wave_sync();
+                bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x,
+                                      at, lane);
+                bmdn_r1<VT, BT, COMP>(A, DENSE, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], 0, x, at, lane);
+                wave_sync();
+            }
+            for (int r = lane; r < rows; r += kWave) {
+                const int64_t gr = (int64_t)B.light[p0 + (uint32_t)(r >> A.k)] * s + (r & (s - 1));
+                if (gr < A.m) {
+                    double a = acc.hi[r];
+                    if constexpr (COMP) a += acc.lo[r];
+                    y[gr] = (beta == 0.0) ? alpha * a : alpha * a + beta * y[gr];
+                }
+            }
+            wave_sync();
+            return;
+        }
 #pragma unroll 1
         for (uint32_t p = p0; p < p1; p++) {
             const int64_t I = B.light ? (int64_t)__builtin_amdgcn_readfirstlane((int)B.light[p]) : (int64_t)p;
@@ -453,7 +508,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
                 wave_sync();
             }
             coo_r1<VT, BT, IT, COMP, ZS, CH>(A, A.sptr[COO][I], A.sptr[COO][I + 1], A.eptr[COO][I], x, acc, s_end, s_bc,
-                                        s_st, lane);
+                                             s_st, s_to, lane);
             csr_rows_r1<VT, BT, RT, COMP>(A, A.sptr[CSR][I], A.sptr[CSR][I + 1], A.eptr[CSR][I], x, acc, s_st, s_bc,
                                           lane);
             wave_sync();
@@ -475,7 +530,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         const uint32_t *U = B.units + u * 4;
         const uint32_t sc = U[0] & 3u, b0 = U[1], b1 = U[2], e0 = U[3];
         if (sc == COO)
-            coo_r1<VT, BT, IT, COMP, ZS, CH>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, lane);
+            coo_r1<VT, BT, IT, COMP, ZS, CH>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, s_to, lane);
         else if (sc == CSR)
             csr_rows_r1<VT, BT, RT, COMP>(A, b0, b1, e0, x, acc, s_st, s_bc, lane);
         else
@@ -500,10 +555,13 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
                                                     double alpha, double beta, Balance B)
 {
     constexpr int kWaves = 4;
-    __shared__ double s_acc[kWaves][256 * (COMP ? 2 : 1)];
+    // s rows, or a tile's (kTileRows; kTileRowsComp compensated, lo half at +256)
+    static_assert(kTileRowsComp == 256 && kTileRows >= 256, "accumulator layout");
+    __shared__ double s_acc[kWaves][COMP ? 512 : kTileRows];
     __shared__ uint32_t s_end[kWaves][kWave];
     __shared__ uint32_t s_bc[kWaves][kWave];
     __shared__ uint32_t s_st[kWaves][kWave];
+    __shared__ uint32_t s_to[kWaves][kWave];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     int64_t item;
     if (B.order) {  // column panels: the slot table already places items on XCDs
@@ -515,7 +573,7 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     }
     if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
     spmv_item<VT, BT, IT, RT, COMP, ZS, CH>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave], s_bc[wave],
-                                            s_st[wave], lane);
+                                            s_st[wave], s_to[wave], lane);
 }
 
 // ===========================================================================
@@ -670,10 +728,11 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
                                                     double alpha, double beta, unsigned long long *cyc)
 {
     constexpr int kWaves = 4;
-    __shared__ double s_acc[kWaves][256];
+    __shared__ double s_acc[kWaves][kTileRows];
     __shared__ uint32_t s_end[kWaves][kWave];
     __shared__ uint32_t s_bc[kWaves][kWave];
     __shared__ uint32_t s_st[kWaves][kWave];
+    __shared__ uint32_t s_to[kWaves][kWave];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * kWaves;
 #pragma unroll 1
@@ -686,7 +745,7 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
         long long t0 = 0;
         if constexpr (TIMED) t0 = clock64();
         spmv_item<double, BT, IT, RT, false>(e.A, e.B, g - e.item0, e.x, e.y, alpha, beta, s_acc[wave],
-                                             s_end[wave], s_bc[wave], s_st[wave], lane);
+                                             s_end[wave], s_bc[wave], s_st[wave], s_to[wave], lane);
         if constexpr (TIMED) {
             const long long t1 = clock64();
             if (lane == 0) atomicAdd(&cyc[mat], (unsigned long long)(t1 - t0));

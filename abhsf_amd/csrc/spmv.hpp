@@ -21,7 +21,24 @@ struct Balance {
     // runs on XCD b % kXcdCount, so items are grouped by panel = XCD.
     const uint32_t *order = nullptr;  // [n_slots]
     int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0;
+    // Tiled variant: light item k holds the strips light[pack[k] .. pack[k+1])
+    // (a tile of COO-only strips, or one strip), accumulated together (their
+    // rows (position << k) + lrow; kTileRows rows at most, kTileRowsComp with
+    // FP32 values); its COO blocks are coo_rng[k] = {b0, b1, first value},
+    // in (block column, strip) order.
+    bool tiled = false;
+    const uint32_t *coo_rng = nullptr;  // [n_light][3]
 };
+
+#ifndef ABHSF_TILE_WORK
+#define ABHSF_TILE_WORK 8192
+#endif
+#ifndef ABHSF_TILE_ROWS
+#define ABHSF_TILE_ROWS 512
+#endif
+constexpr int kTileRows = ABHSF_TILE_ROWS;      // rows per tile at most (FP64 values)
+constexpr int kTileRowsComp = 256;              // with FP32 values (two accumulators per row)
+constexpr int64_t kTileWork = ABHSF_TILE_WORK;  // entries per tile at most
 
 constexpr int kXcdCount = 8;            // XCDs of an MI355X; workgroups go round-robin
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;

@@ -22,6 +22,12 @@ struct Storage {
     // with column panels, the slot table places each item on its panel's XCD)
     DevBuf bal_light, bal_pack, bal_units, bal_heavy, bal_part, bal_order;
     int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0;
+    // tiled variant: light items are tiles of consecutive strips whose COO
+    // blocks are stored in (block column, strip) order; COO z1 holds z - 1 in
+    // its low coo_zb bits and the strip within the tile above them
+    bool tiled = false;
+    int coo_zb = 16;
+    DevBuf bal_coo_rng;  // tiled: [n_light][3] COO {first block, end block, first value} of each item
     // footprints (bits) under both cost models
     uint64_t smf_preproc = 0, smf_msmf = 0;
     double build_ms = 0.0;
@@ -34,8 +40,13 @@ struct Storage {
     // columns are also cut into `panels` column panels of equal work: a strip
     // whose blocks span several panels is split at the panel boundaries, and
     // every work item is placed on the XCD of its panel (panels must equal the
-    // XCD count, kXcdCount).
-    void build_balance(int64_t unit, hipStream_t st, int panels = 1);
+    // XCD count, kXcdCount).  With tile_rows > 0 (tiled variant; panels must
+    // be 1) light strips whose entries are all COO are gathered, in strip
+    // order, into tiles of at most tile_rows rows and kTileWork entries (other
+    // light strips stay items of their own), and the COO group is reordered so
+    // that every item's blocks are contiguous, a tile's sorted by (block
+    // column, strip).
+    void build_balance(int64_t unit, hipStream_t st, int panels = 1, int tile_rows = 0);
     uint64_t stored_bytes() const;
     uint64_t value_bytes() const;
     void set_shape(int64_t m_, int64_t n_, int s_);

@@ -57,7 +57,7 @@ def _addr(a) -> int:
 _POLICY = {"preproc": capi.POLICY_PREPROC, "msmf": capi.POLICY_MSMF}
 _PREC = {"f64": capi.F64, "f32": capi.F32}
 _VAR = {"auto": capi.VARIANT_AUTO, "strip": capi.VARIANT_STRIP, "balanced": capi.VARIANT_BALANCED,
-        "panel": capi.VARIANT_PANEL}
+        "panel": capi.VARIANT_PANEL, "tiled": capi.VARIANT_TILED}
 
 
 class AbhsfMatrix:

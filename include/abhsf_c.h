@@ -68,9 +68,15 @@ enum {
     ABHSF_VARIANT_AUTO = 0,
     ABHSF_VARIANT_STRIP = 1,    /* one wavefront per strip */
     ABHSF_VARIANT_BALANCED = 2, /* heavy strips split over several wavefronts */
-    ABHSF_VARIANT_PANEL = 3     /* balanced, plus 8 column panels of equal work, each
+    ABHSF_VARIANT_PANEL = 3,    /* balanced, plus 8 column panels of equal work, each
                                    processed on one XCD so that its slice of x stays in
                                    that XCD's L2; strips spanning panels are split there */
+    ABHSF_VARIANT_TILED = 4     /* balanced, plus runs of consecutive light strips (up to
+                                   512 rows, 256 with F32 values) processed as one tile
+                                   whose COO blocks are stored in block-column order, so
+                                   that all wavefronts sweep x together.  SpMV (single
+                                   and batched) only: SpMM, symmetric storage and export
+                                   refuse a tiled handle (ABHSF_ERR_UNSUPPORTED) */
 };
 
 /* Symmetry of the stored matrix (matrix_symmetry_t, include/abhsf/utils/

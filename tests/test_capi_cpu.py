@@ -35,7 +35,8 @@ def test_library_has_gfx950_code_object():
 @pytest.mark.parametrize("bs,policy,prec,variant,ok", [
     (64, 0, 0, 0, True), (2, 1, 1, 2, True), (256, 0, 0, 1, True),
     (1, 0, 0, 0, False), (48, 0, 0, 0, False), (512, 0, 0, 0, False),
-    (64, 0, 0, 3, True), (64, 2, 0, 0, False), (64, 0, 2, 0, False), (64, 0, 0, 4, False),
+    (64, 0, 0, 3, True), (64, 0, 0, 4, True), (64, 2, 0, 0, False), (64, 0, 2, 0, False),
+    (64, 0, 0, 5, False),
 ])
 def test_create_validates_options(bs, policy, prec, variant, ok):
     opts = capi.Options(bs, policy, prec, variant, 0, 0, None)

@@ -41,7 +41,7 @@ def test_batched_launch_eager_and_graph(gpu):
     st = hip.Stream()
     Ms, xs, ys, refs, y0s = [], [], [], [], []
     for i, (A, s) in enumerate(members):
-        M = AbhsfMatrix(block_size=s, variant=("balanced", "strip", "panel")[i % 3], stream=st)
+        M = AbhsfMatrix(block_size=s, variant=("balanced", "strip", "panel", "tiled")[(i // 4) % 4], stream=st)
         M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
         x = np.random.default_rng(i).uniform(-1, 1, A.n)
         y0 = np.random.default_rng(i + 100).uniform(-1, 1, A.m)
