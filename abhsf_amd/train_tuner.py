@@ -1,6 +1,6 @@
 """Train the learned tuner on the GPU and judge it against exhaustive search.
 
-  python -m abhsf_amd.train_tuner --out DIR [--steps 4000] [--mode latency|batch]
+  python -m abhsf_amd.train_tuner --out DIR [--mode latency|batch] [--steps N --lr R --members K]
 
 Mode latency (single-matrix SpMV):
 
@@ -44,17 +44,23 @@ from . import hip, synth, tuner
 def corpus(split: str):
     """(name, generator) pairs."""
     if split == "train":
-        out = [(f"lap_{g}", (lambda g=g: synth.laplacian2d(g))) for g in (256, 700, 1500)]
-        out += [(f"stencil_{g}", (lambda g=g: synth.stencil27(g))) for g in (40, 80, 128)]
+        out = [(f"lap_{g}", (lambda g=g: synth.laplacian2d(g))) for g in (256, 450, 700, 1500, 2400)]
+        out += [(f"stencil_{g}", (lambda g=g: synth.stencil27(g))) for g in (40, 64, 80, 128, 150)]
         for n in (1 << 17, 1 << 19, 1 << 21):
             for pd, pb in ((0.2, 0.3), (0.0, 0.0), (1.0, 0.0), (0.0, 1.0), (0.5, 0.5)):
                 out.append((f"mix_{n}_{pd}_{pb}", (lambda n=n, pd=pd, pb=pb: synth.block_mix(n, 11, pd, pb))))
-        for sc, ef in ((16, 8), (18, 16), (20, 8), (20, 16), (21, 16)):
+        for n in (1 << 18, 1 << 20, 1 << 22):
+            for pd, pb in ((0.1, 0.1), (0.3, 0.6), (0.2, 0.3)):
+                out.append((f"mix_{n}_{pd}_{pb}", (lambda n=n, pd=pd, pb=pb: synth.block_mix(n, 12, pd, pb))))
+        for sc, ef in ((16, 8), (17, 16), (18, 16), (18, 32), (19, 8), (20, 8), (20, 16), (21, 16), (22, 8),
+                       (22, 16)):
             out.append((f"rmat_{sc}_{ef}", (lambda sc=sc, ef=ef: synth.rmat(sc, ef, seed=11))))
-        for n, bw in ((100000, 4), (100000, 32), (1000000, 8), (1000000, 64)):
+        for n, bw in ((100000, 4), (100000, 32), (200000, 128), (500000, 16), (1000000, 8), (1000000, 64),
+                      (2000000, 4)):
             out.append((f"banded_{n}_{bw}", (lambda n=n, bw=bw: synth.banded(n, bw, 11))))
         for n, bs, nbr, d in ((100000, 8, 4, 0.3), (100000, 16, 2, 0.1), (1000000, 4, 8, 0.5),
-                              (1000000, 32, 4, 0.05), (500000, 16, 8, 0.2)):
+                              (1000000, 32, 4, 0.05), (500000, 16, 8, 0.2), (2000000, 8, 2, 0.4),
+                              (300000, 64, 2, 0.02)):
             out.append((f"rblk_{n}_{bs}_{nbr}_{d}", (lambda n=n, bs=bs, nbr=nbr, d=d:
                                                     synth.random_blocks(n, bs, nbr, d, 11))))
         return out
@@ -158,7 +164,7 @@ def main_batch(args, log):
     Xf = Xtr.reshape(-1, tuner.NFEAT)
     ytr = np.log(np.maximum(ctr, 1.0) / ntr[:, None]).reshape(-1)
     w = np.repeat(np.sqrt(ntr / ntr.mean()), len(cands))
-    model, losses = tuner.train(Xf, ytr, w=w, steps=args.steps, lr=args.lr)
+    model, losses = tuner.train(Xf, ytr, w=w, steps=args.steps, lr=args.lr, members=args.members)
     log(f"trained: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {model.meta['train_s']:.1f} s")
     model.meta["labels"] = "log(cycles per entry inside a batched launch)"
     model.meta["variants"] = list(tuner.BATCH_VARIANTS)
@@ -204,10 +210,18 @@ def main_batch(args, log):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    # latency defaults chosen by 5-fold cross-validation over the training
+    # matrices (grid steps {300 .. 6000} x lr {1e-3, 3e-3, 1e-2}, mean
+    # regret); the batch set is 8x larger and keeps the longer schedule
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--lr", type=float, default=None)
+    ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--mode", choices=["latency", "batch"], default="latency")
     args = ap.parse_args(argv)
+    if args.steps is None:
+        args.steps = 4000 if args.mode == "batch" else 700
+    if args.lr is None:
+        args.lr = 2e-3 if args.mode == "batch" else 1e-2
     os.makedirs(args.out, exist_ok=True)
 
     def log(msg):
@@ -220,7 +234,7 @@ def main(argv=None):
     Xtr, ytr, ntr, ttr = collect("train", stream, log)
     Xte, yte, nte, tte = collect("test", stream, log)
     np.savez(os.path.join(args.out, "tuner_dataset.npz"), Xtr=Xtr, ytr=ytr, Xte=Xte, yte=yte)
-    model, losses = tuner.train(Xtr, ytr, steps=args.steps, lr=args.lr)
+    model, losses = tuner.train(Xtr, ytr, steps=args.steps, lr=args.lr, members=args.members)
     log(f"trained: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {model.meta['train_s']:.1f} s")
     model.save(os.path.join(args.out, "tuner_model.json"))
 

@@ -18,7 +18,9 @@ Features (the statistics block_statistics/ computes, block_stats.cpp:78-100,
               entries in each scheme, log2 predicted stored bytes per entry,
               strip-work stdev / mean, log2 (max strip work / mean),
               fraction of entries in strips above the split threshold,
-              log2 strips, variant flags (balanced, panel)
+              log2 strips, fraction of entries in light strips whose blocks
+              are all COO (what the tiled variant gathers into tiles),
+              variant flags (balanced, panel, delta, tiled)
 Block-size histograms come from abhsf_block_hist on a sample of strips.
 """
 from __future__ import annotations
@@ -35,7 +37,7 @@ from . import capi, hip
 
 NAME = "mlp-2x64"
 S_CANDIDATES = (8, 16, 32, 64, 128, 256)
-VARIANTS = ("strip", "balanced", "panel")
+VARIANTS = ("strip", "balanced", "panel", "tiled")
 DELTA = (0, "delta")  # the delta-coded coordinate format (no block size), one more candidate
 HIDDEN = 64
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -119,7 +121,37 @@ def unit_entries(s):
     return max(2048, 8 * s)
 
 
-def size_features(rowptr, m, n, s, hist, nblk, sampled):
+def coo_max_z(s: int) -> int:
+    """Largest z whose block is stored as COO (the scheme is monotone in z)."""
+    z = 1
+    while z < s * s and choose(s, z + 1) == 0:
+        z += 1
+    return z
+
+
+def tile_fraction(rowptr, col, m, s, stride):
+    """Fraction of the sampled strips' entries (every stride-th strip) that
+    lie in light strips whose blocks are all COO."""
+    k = s.bit_length() - 1
+    M = -(-m // s)
+    strips = np.arange(0, M, stride, dtype=np.int64)
+    e0 = rowptr[strips * s]
+    e1 = rowptr[np.minimum(strips * s + s, m)]
+    cnt = (e1 - e0).astype(np.int64)
+    tot = int(cnt.sum())
+    if tot == 0:
+        return 0.0
+    idx = np.repeat(e0 - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt) + np.arange(tot, dtype=np.int64)
+    sid = np.repeat(np.arange(strips.size, dtype=np.int64), cnt)
+    key = (sid << np.int64(32)) | (np.asarray(col, dtype=np.int64)[idx] >> np.int64(k))
+    u, z = np.unique(key, return_counts=True)
+    mixed = np.zeros(strips.size, dtype=bool)
+    mixed[(u >> np.int64(32))[z > coo_max_z(s)]] = True
+    pure = ~mixed & (cnt <= 2 * unit_entries(s))
+    return float(cnt[pure].sum()) / tot
+
+
+def size_features(rowptr, m, n, s, hist, nblk, sampled, tile_frac=0.0):
     nnz = int(rowptr[-1])
     scale = nnz / max(sampled, 1)
     nb = [0.0] * 4
@@ -144,7 +176,7 @@ def size_features(rowptr, m, n, s, hist, nblk, sampled):
         "log2_bytes_per_entry": math.log2(max(stored, 1) / max(nnz, 1)),
         "strip_cv": float(work.std()) / mean, "log2_max_strip": math.log2(max(float(work.max()), 1.0) / mean),
         "heavy_frac": float(work[work > H].sum()) / max(float(work.sum()), 1.0), "log2_M": math.log2(M),
-        "stored_bytes": stored,
+        "tile_frac": tile_frac, "stored_bytes": stored,
     }
 
 
@@ -153,12 +185,12 @@ def feature_vector(g, sf, variant):
         g["log2_n"], g["log2_nnz"], g["log2_mean_row"], g["row_cv"], g["log2_max_row"], g["empty_rows"],
         sf["log2_s"], sf["log2_blk_per_entry"], *sf["blk_frac"], *sf["ent_frac"],
         sf["log2_bytes_per_entry"], sf["strip_cv"], sf["log2_max_strip"], sf["heavy_frac"], sf["log2_M"],
-        1.0 if variant == "balanced" else 0.0, 1.0 if variant == "panel" else 0.0,
-        1.0 if variant == "delta" else 0.0,
+        sf["tile_frac"], 1.0 if variant == "balanced" else 0.0, 1.0 if variant == "panel" else 0.0,
+        1.0 if variant == "delta" else 0.0, 1.0 if variant == "tiled" else 0.0,
     ], dtype=np.float64)
 
 
-NFEAT = 6 + 18  # 6 global + 18 per candidate (fractions are 4 values each)
+NFEAT = 6 + 20  # 6 global + 20 per candidate (fractions are 4 values each)
 
 
 def delta_features(rowptr, col, n):
@@ -179,14 +211,15 @@ def delta_features(rowptr, col, n):
     return {
         "log2_s": 0.0, "log2_blk_per_entry": 0.0, "blk_frac": [0.0] * 4, "ent_frac": [c / tot for c in cls],
         "log2_bytes_per_entry": math.log2(max(stored, 1) / tot), "strip_cv": 0.0, "log2_max_strip": 0.0,
-        "heavy_frac": 0.0, "log2_M": math.log2(max(chunks, 1)), "stored_bytes": stored,
+        "heavy_frac": 0.0, "log2_M": math.log2(max(chunks, 1)), "tile_frac": 0.0, "stored_bytes": stored,
     }
 
 
-# Batched launches use the balanced variant only: inside one launch a whole
-# heavy strip on one wave (strip variant) sets the launch's tail, which the
-# per-member cycle label cannot see, and column panels only add partial rows.
-BATCH_VARIANTS = ("balanced",)
+# Batched launches use the balanced and tiled variants: inside one launch a
+# whole heavy strip on one wave (strip variant) sets the launch's tail, which
+# the per-member cycle label cannot see, and column panels only add partial
+# rows.
+BATCH_VARIANTS = ("balanced", "tiled")
 
 
 def candidates(variants=VARIANTS + ("delta",)):
@@ -206,7 +239,8 @@ def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
             hist, nblk, ns = block_hist(d_rowptr, d_col, A.m, A.n, s, stride, True, A.nnz, stream)
         else:
             hist, nblk, ns = block_hist(A.rowptr, A.col, A.m, A.n, s, stride, False)
-        per_s[s] = size_features(A.rowptr, A.m, A.n, s, hist, nblk, ns)
+        per_s[s] = size_features(A.rowptr, A.m, A.n, s, hist, nblk, ns,
+                                 tile_fraction(A.rowptr, A.col, A.m, s, stride))
     for s, v in candidates():
         sf = per_s[s] if v != "delta" else delta_features(A.rowptr, A.col, A.n)
         rows.append(feature_vector(g, sf, v))
@@ -309,6 +343,10 @@ def init_theta(F, seed=0):
 
 
 class Model:
+    """An ensemble of MLPs sharing the input normalisation: theta is one
+    parameter vector or a [members, P] array; predictions are the members'
+    mean."""
+
     def __init__(self, theta, mu, sd, ymu=0.0, meta=None):
         self.theta = np.asarray(theta, dtype=np.float32)
         self.mu = np.asarray(mu, dtype=np.float64)
@@ -322,7 +360,7 @@ class Model:
         return cls(d["theta"], d["mu"], d["sd"], d.get("ymu", 0.0), d.get("meta"))
 
     def save(self, path=MODEL_PATH):
-        json.dump({"theta": [float(v) for v in self.theta], "mu": list(map(float, self.mu)),
+        json.dump({"theta": self.theta.astype(np.float64).tolist(), "mu": list(map(float, self.mu)),
                    "sd": list(map(float, self.sd)), "ymu": self.ymu, "meta": self.meta},
                   open(path, "w"))
 
@@ -338,20 +376,33 @@ class Model:
         return ((X - self.mu) / self.sd).astype(np.float32)
 
     def predict(self, X):
-        """log(ms per entry) predictions, computed by the GPU forward kernel."""
+        """log(ms per entry) predictions, computed by the GPU forward kernel
+        (mean over the ensemble members)."""
         Xn = self.normalize(X)
         B, F = Xn.shape
-        dt = hip.DeviceArray.from_numpy(self.theta)
         dx = hip.DeviceArray.from_numpy(Xn)
         h1, h2, out = hip.DeviceArray((B, HIDDEN), np.float32), hip.DeviceArray((B, HIDDEN), np.float32), \
             hip.DeviceArray((B,), np.float32)
-        capi.call("abhsf_mlp_forward", dt.ptr, F, dx.ptr, B, h1.ptr, h2.ptr, out.ptr, None)
-        hip.synchronize()
-        return out.numpy().astype(np.float64) + self.ymu
+        acc = np.zeros(B)
+        members = self.theta.reshape(-1, self.theta.shape[-1])
+        for th in members:
+            dt = hip.DeviceArray.from_numpy(np.ascontiguousarray(th))
+            capi.call("abhsf_mlp_forward", dt.ptr, F, dx.ptr, B, h1.ptr, h2.ptr, out.ptr, None)
+            hip.synchronize()
+            acc += out.numpy().astype(np.float64)
+        return acc / len(members) + self.ymu
 
 
-def train(X, y, w=None, steps=3000, lr=3e-3, seed=0, mu=None, sd=None):
-    """Fit the MLP on the GPU (full-batch Adam by abhsf_mlp_train)."""
+def train(X, y, w=None, steps=700, lr=1e-2, seed=0, mu=None, sd=None, members=1):
+    """Fit the MLP on the GPU (full-batch Adam by abhsf_mlp_train); with
+    members > 1 an ensemble of that many MLPs (seeds seed, seed + 1, ...)."""
+    if members > 1:
+        fits = [train(X, y, w, steps, lr, seed + i, mu, sd) for i in range(members)]
+        model = fits[0][0]
+        model.theta = np.stack([m.theta for m, _ in fits])
+        model.meta["members"] = members
+        model.meta["train_s"] = sum(m.meta["train_s"] for m, _ in fits)
+        return model, np.mean([l for _, l in fits], axis=0)
     X = np.asarray(X, dtype=np.float64)
     mu = X.mean(0) if mu is None else mu
     sd = X.std(0) if sd is None else sd

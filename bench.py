@@ -4,7 +4,7 @@ Default workload (the largest single-GPU configuration the metric is quoted
 on): block-structured synthetic matrix, n = 2^22, 32-row strips drawn 20%
 three dense 32x32 blocks / 30% block-tridiagonal of 8x8 sub-blocks / 50% 12
 uniform columns per row, values N(0,1) FP64 (seed 0), x ~ U[-1,1] (seed 0);
-ABHSF at s in {16, 32, 64, 128}.  A step is one SpMV y += A*x (the reference
+ABHSF at s in {16, 32, 64, 128}, balanced and tiled variants.  A step is one SpMV y += A*x (the reference
 harness's accumulating call, spmv/unified/spmv.cpp:1222-1231) with all inputs
 resident in HBM.
 
@@ -253,7 +253,7 @@ def run_block_mix(args, hip):
         cands = [pick]
     else:
         pick, tname = tuned_pick(A, s_list, hip)
-        cands = [(s, "balanced") for s in s_list]
+        cands = [(s, v) for s in s_list for v in ("balanced", "tiled")]
         if pick is not None and pick not in cands:
             cands.append(pick)
     res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
@@ -271,8 +271,8 @@ def run_block_mix(args, hip):
     out["roofline"] = roofline(best, f"block_mix/{pick[0]}/{pick[1]}")
     out["plain_s64"] = None if plain is None else {"gflops": plain["gflops"], "gbs": plain["gbs"]}
     out["tuned_vs_plain"] = None if plain is None else round(best["gflops"] / plain["gflops"], 3)
-    out["exhaustive_best"] = {"s": max(res, key=lambda k: res[k]["gflops"])[0],
-                              "gflops": max(r["gflops"] for r in res.values())}
+    top = max(res, key=lambda k: res[k]["gflops"])
+    out["exhaustive_best"] = {"s": top[0], "variant": top[1], "gflops": res[top]["gflops"]}
     out["per_s"] = {f"{s}/{v}": r for (s, v), r in res.items()}
     if not args.no_calib:
         out["copy_calibration"] = calib_copy_gbs(hip)
@@ -446,7 +446,7 @@ def main():
     ap.add_argument("--workload", default="block_mix", choices=["block_mix", "rmat", "spmm", "batch"])
     ap.add_argument("--s", default=None, help="block sizes to measure (block_mix default 16,32,64,128; "
                     "rmat default all tuner candidates)")
-    ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel", "delta"],
+    ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel", "tiled", "delta"],
                     help="block_mix / rmat: measure only (first --s, VARIANT), no tuner (profiler passes)")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--spmm-s", type=int, default=64)

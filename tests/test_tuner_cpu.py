@@ -72,6 +72,42 @@ def test_feature_vector_length_matches_nfeat():
     g = {"log2_n": 1.0, "log2_nnz": 1.0, "log2_mean_row": 0.0, "row_cv": 0.0, "log2_max_row": 0.0,
          "empty_rows": 0.0}
     sf = {"log2_s": 3.0, "log2_blk_per_entry": 0.0, "blk_frac": [1, 0, 0, 0], "ent_frac": [1, 0, 0, 0],
-          "log2_bytes_per_entry": 3.5, "strip_cv": 0.0, "log2_max_strip": 0.0, "heavy_frac": 0.0, "log2_M": 4.0}
-    for v in tuner.VARIANTS:
+          "log2_bytes_per_entry": 3.5, "strip_cv": 0.0, "log2_max_strip": 0.0, "heavy_frac": 0.0, "log2_M": 4.0,
+          "tile_frac": 0.5}
+    for v in tuner.VARIANTS + ("delta",):
         assert tuner.feature_vector(g, sf, v).shape == (tuner.NFEAT,)
+
+
+@pytest.mark.parametrize("s", [2, 4, 8, 16, 32, 64, 128, 256])
+def test_coo_blocks_hold_at_most_2s_entries(s):
+    """COO is kept while 2kz + log s^2 <= kz + (s+1) log s^2 (the tie at
+    z = 2s keeps COO: strict < in preprocessing/test.cpp:55-77), so COO
+    blocks hold z <= 2s entries and z - 1 fits k + 1 bits: the u16 z1 of the
+    tiled variant has 15 - k bits left for the strip index."""
+    k = s.bit_length() - 1
+    zc = tuner.coo_max_z(s)
+    assert tuner.choose(s, zc) == 0 and tuner.choose(s, zc + 1) != 0
+    assert all(tuner.choose(s, z) == 0 for z in range(1, zc + 1))
+    assert zc <= 2 * s and zc - 1 < (1 << (k + 1))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tile_fraction_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    A = synth.block_mix(1 << 11, seed, 0.2, 0.3) if seed % 2 else synth.rmat(11, 8, seed=seed)
+    for s in (4, 16, 64):
+        for stride in (1, 3):
+            got = tuner.tile_fraction(A.rowptr, A.col, A.m, s, stride)
+            M = -(-A.m // s)
+            tot = pure = 0
+            for I in range(0, M, stride):
+                r0, r1 = I * s, min(I * s + s, A.m)
+                cols = A.col[A.rowptr[r0]:A.rowptr[r1]]
+                zs = {}
+                for c in cols:
+                    zs[int(c) // s] = zs.get(int(c) // s, 0) + 1
+                w = len(cols)
+                tot += w
+                if w <= 2 * tuner.unit_entries(s) and all(tuner.choose(s, z) == 0 for z in zs.values()):
+                    pure += w
+            assert got == pytest.approx(pure / tot if tot else 0.0), (s, stride)
