@@ -44,7 +44,6 @@ struct abhsf_batch {
     std::vector<abhsf::BatchEntry> host;
     abhsf::DevBuf table, heavy, item_mat;
     int64_t total_items = 0, nheavy = 0;
-    int64_t code_begin[abhsf::kBatchCodes] = {}, code_end[abhsf::kBatchCodes] = {};
     bool vectors_set = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -422,17 +421,21 @@ abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *
             for (int64_t q = 0; q < e.B.n_heavy; q++) hv.insert(hv.end(), {(uint32_t)i, (uint32_t)q});
             b->host.push_back(e);
         }
-        // number the items grouped by index-type combination (one kernel per
-        // combination, each with the register budget of its own variant)
+        // number the items member by member, longest first: members in
+        // decreasing order of entries per work item (one kernel serves all
+        // index-type combinations; the grid-stride loop starts the items in
+        // this order, so the long ones do not form the launch's tail)
+        std::vector<size_t> ord(b->host.size());
+        std::iota(ord.begin(), ord.end(), (size_t)0);
+        auto per_item = [&](size_t i) {
+            const auto &e = b->host[i];
+            return (double)mats[i]->S.nnz / (double)std::max<int64_t>(1, e.B.n_light + e.B.n_units);
+        };
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t c) { return per_item(a) > per_item(c); });
         int64_t item = 0;
-        for (int c = 0; c < abhsf::kBatchCodes; c++) {
-            b->code_begin[c] = item;
-            for (auto &e : b->host)
-                if (e.code == c) {
-                    e.item0 = item;
-                    item += e.B.n_light + e.B.n_units;
-                }
-            b->code_end[c] = item;
+        for (size_t i : ord) {
+            b->host[i].item0 = item;
+            item += b->host[i].B.n_light + b->host[i].B.n_units;
         }
         b->total_items = item;
         b->nheavy = (int64_t)hv.size() / 2;
@@ -475,8 +478,8 @@ abhsf_status abhsf_batch_spmv(abhsf_batch *b, double alpha, double beta, int32_t
         ABHSF_REQUIRE(b, ABHSF_ERR_INVALID, "null batch");
         ABHSF_REQUIRE(b->vectors_set, ABHSF_ERR_STATE, "abhsf_batch_set_vectors not called");
         auto launch = [&] {
-            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->code_begin,
-                              b->code_end, b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream);
+            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
+                              b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream);
         };
         if (!use_graph) {
             launch();
@@ -521,8 +524,8 @@ abhsf_status abhsf_batch_profile(abhsf_batch *b, double alpha, double beta, uint
         const size_t nm = b->host.size();
         abhsf::DevBuf cyc(nm * sizeof(unsigned long long));
         ABHSF_HIP(hipMemsetAsync(cyc.get(), 0, cyc.bytes(), b->stream));
-        abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->code_begin,
-                          b->code_end, b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream,
+        abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
+                          b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream,
                           cyc.get<unsigned long long>());
         ABHSF_HIP(hipMemcpyAsync(cycles, cyc.get(), cyc.bytes(), hipMemcpyDeviceToHost, b->stream));
         ABHSF_HIP(hipStreamSynchronize(b->stream));

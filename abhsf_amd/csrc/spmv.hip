@@ -529,12 +529,17 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         const int64_t u = item - B.n_light;
         const uint32_t *U = B.units + u * 4;
         const uint32_t sc = U[0] & 3u, b0 = U[1], b1 = U[2], e0 = U[3];
+        // constant scheme arguments: the descriptor's per-scheme arrays are
+        // then indexed statically (a runtime index would force a local copy
+        // of the descriptor, as in the batched kernel, into scratch memory)
         if (sc == COO)
             coo_r1<VT, BT, IT, COMP, ZS, CH>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, s_to, lane);
         else if (sc == CSR)
             csr_rows_r1<VT, BT, RT, COMP>(A, b0, b1, e0, x, acc, s_st, s_bc, lane);
+        else if (sc == BITMAP)
+            bmdn_r1<VT, BT, COMP>(A, BITMAP, b0, b1, e0, x, acc, lane);
         else
-            bmdn_r1<VT, BT, COMP>(A, (int)sc, b0, b1, e0, x, acc, lane);
+            bmdn_r1<VT, BT, COMP>(A, DENSE, b0, b1, e0, x, acc, lane);
         wave_sync();
         double *part = B.part + u * (size_t)s * (COMP ? 2 : 1);
         for (int r = lane; r < s; r += kWave) {
@@ -702,30 +707,68 @@ __global__ __launch_bounds__(256) void k_spmv_trans(DevMat A, Balance B, const d
     }
 }
 
-// Copy of a wave-uniform table entry, read through the constant address
-// space (the table does not change during a launch) so that the compiler
-// uses scalar loads and keeps the fields in SGPRs.
+// A wave-uniform table field, read through the constant address space (the
+// table does not change during a launch): a scalar load into SGPRs.
 template <typename T>
-__device__ inline T load_uniform(const T *p)
+__device__ inline T ld_const(const T *p)
 {
-    static_assert(sizeof(T) % 4 == 0, "word-sized struct expected");
-    T out;
-    const auto *src = (const __attribute__((address_space(4))) uint32_t *)p;
-    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
-#pragma unroll
-    for (size_t i = 0; i < sizeof(T) / 4; i++) dst[i] = src[i];
-    return out;
+    return *(const __attribute__((address_space(4))) T *)p;
 }
 
-// Many matrices in one launch: the waves take the global items [g0, g1) of
-// one index-type combination, wave w items g0 + w, g0 + w + W, ... (W = all
-// waves of the grid); item_mat[g] names the matrix of item g.  FP64 values.
-// TIMED: each wave also adds the shader-clock cycles of its item to
+// Copy of a batch table entry, field by field (a word-wise copy through a
+// cast pointer kept the whole entry in per-lane scratch memory).
+__device__ inline BatchEntry load_entry(const BatchEntry *p)
+{
+    BatchEntry e;
+    e.A.m = ld_const(&p->A.m);
+    e.A.n = ld_const(&p->A.n);
+    e.A.M = ld_const(&p->A.M);
+    e.A.N = ld_const(&p->A.N);
+    e.A.s = ld_const(&p->A.s);
+    e.A.k = ld_const(&p->A.k);
+    e.A.W = ld_const(&p->A.W);
+    e.A.zb = ld_const(&p->A.zb);
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        e.A.sptr[x] = ld_const(&p->A.sptr[x]);
+        e.A.eptr[x] = ld_const(&p->A.eptr[x]);
+        e.A.bcol[x] = ld_const(&p->A.bcol[x]);
+        e.A.z1[x] = ld_const(&p->A.z1[x]);
+        e.A.val[x] = ld_const(&p->A.val[x]);
+    }
+    e.A.coo_idx = ld_const(&p->A.coo_idx);
+    e.A.csr_rowptr = ld_const(&p->A.csr_rowptr);
+    e.A.csr_col = ld_const(&p->A.csr_col);
+    e.A.bm_words = ld_const(&p->A.bm_words);
+    e.B.light = ld_const(&p->B.light);
+    e.B.pack = ld_const(&p->B.pack);
+    e.B.units = ld_const(&p->B.units);
+    e.B.heavy = ld_const(&p->B.heavy);
+    e.B.part = ld_const(&p->B.part);
+    e.B.order = ld_const(&p->B.order);
+    e.B.n_light = ld_const(&p->B.n_light);
+    e.B.n_units = ld_const(&p->B.n_units);
+    e.B.n_heavy = ld_const(&p->B.n_heavy);
+    e.B.n_slots = ld_const(&p->B.n_slots);
+    e.B.tiled = ld_const(&p->B.tiled);
+    e.B.coo_rng = ld_const(&p->B.coo_rng);
+    e.x = ld_const(&p->x);
+    e.y = ld_const(&p->y);
+    e.item0 = ld_const(&p->item0);
+    e.code = ld_const(&p->code);
+    return e;
+}
+
+// Many matrices in one launch: the waves take the global items [0, G), wave w
+// items w, w + W, ... (W = all waves of the grid); item_mat[g] names the
+// matrix of item g, whose index-type combination (batch_code) selects the
+// decode (a uniform branch), so one launch serves every member.  FP64
+// values.  TIMED: each wave also adds the shader-clock cycles of its item to
 // cyc[matrix] (the per-matrix cost inside a full launch).
-template <typename BT, typename IT, typename RT, bool TIMED>
+template <bool TIMED>
 __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict__ E,
-                                                    const uint32_t *__restrict__ item_mat, int64_t g0, int64_t g1,
-                                                    double alpha, double beta, unsigned long long *cyc)
+                                                    const uint32_t *__restrict__ item_mat, int64_t G, double alpha,
+                                                    double beta, unsigned long long *cyc)
 {
     constexpr int kWaves = 4;
     __shared__ double s_acc[kWaves][kTileRows];
@@ -736,16 +779,43 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * kWaves;
 #pragma unroll 1
-    for (int64_t g = g0 + (int64_t)blockIdx.x * kWaves + wave; g < g1; g += nw) {
+    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < G; g += nw) {
         const int mat = __builtin_amdgcn_readfirstlane((int)item_mat[g]);
-        // the table is read-only during the launch: read the entry through the
-        // constant address space so its fields are scalar loads into SGPRs
-        // (as the single-matrix kernel gets them from its arguments)
-        const BatchEntry e = load_uniform(E + mat);
+        // the table is read-only during the launch: its fields are read through
+        // the constant address space (scalar loads into SGPRs, as the
+        // single-matrix kernel gets them from its arguments)
+        const BatchEntry e = load_entry(E + mat);
         long long t0 = 0;
         if constexpr (TIMED) t0 = clock64();
-        spmv_item<double, BT, IT, RT, false>(e.A, e.B, g - e.item0, e.x, e.y, alpha, beta, s_acc[wave],
-                                             s_end[wave], s_bc[wave], s_st[wave], s_to[wave], lane);
+        const int64_t it = g - e.item0;
+        double *sa = s_acc[wave];
+        uint32_t *se = s_end[wave], *sb = s_bc[wave], *ss = s_st[wave], *so = s_to[wave];
+        switch (e.code) {  // batch_code: (bcol u32 ? 3 : 0) + (s <= 8 ? 0 : s <= 16 ? 1 : 2)
+        case 0:
+            spmv_item<double, uint16_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+                                                                 ss, so, lane);
+            break;
+        case 1:
+            spmv_item<double, uint16_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+                                                                  ss, so, lane);
+            break;
+        case 2:
+            spmv_item<double, uint16_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se,
+                                                                   sb, ss, so, lane);
+            break;
+        case 3:
+            spmv_item<double, uint32_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+                                                                 ss, so, lane);
+            break;
+        case 4:
+            spmv_item<double, uint32_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+                                                                  ss, so, lane);
+            break;
+        default:
+            spmv_item<double, uint32_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se,
+                                                                   sb, ss, so, lane);
+            break;
+        }
         if constexpr (TIMED) {
             const long long t1 = clock64();
             if (lane == 0) atomicAdd(&cyc[mat], (unsigned long long)(t1 - t0));
@@ -1214,35 +1284,18 @@ void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int6
 
 int batch_code(int bcol_bytes, int s) { return (bcol_bytes == 4 ? 3 : 0) + (s <= 8 ? 0 : (s <= 16 ? 1 : 2)); }
 
-template <typename BT, typename IT, typename RT>
-void launch_batch_code(const BatchEntry *dE, const uint32_t *item_mat, int64_t g0, int64_t g1, double alpha,
-                       double beta, hipStream_t st, unsigned long long *cycles)
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t G, const uint32_t *dheavy,
+                int64_t nheavy, double alpha, double beta, hipStream_t st, unsigned long long *cycles)
 {
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((g1 - g0 + 3) / 4, 8192));
-    if (cycles)
-        hipLaunchKernelGGL((k_batch_spmv<BT, IT, RT, true>), dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat,
-                           g0, g1, alpha, beta, cycles);
-    else
-        hipLaunchKernelGGL((k_batch_spmv<BT, IT, RT, false>), dim3((unsigned)grid), dim3(256), 0, st, dE,
-                           item_mat, g0, g1, alpha, beta, cycles);
-    ABHSF_HIP(hipGetLastError());
-}
-
-void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, const int64_t *code_begin,
-                const int64_t *code_end, const uint32_t *dheavy, int64_t nheavy, double alpha, double beta,
-                hipStream_t st, unsigned long long *cycles)
-{
-    for (int c = 0; c < kBatchCodes; c++) {
-        const int64_t g0 = code_begin[c], g1 = code_end[c];
-        if (g1 <= g0) continue;
-        switch (c) {  // batch_code: (bcol u32 ? 3 : 0) + (s <= 8 ? 0 : s <= 16 ? 1 : 2)
-        case 0: launch_batch_code<uint16_t, uint8_t, uint8_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
-        case 1: launch_batch_code<uint16_t, uint8_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
-        case 2: launch_batch_code<uint16_t, uint16_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
-        case 3: launch_batch_code<uint32_t, uint8_t, uint8_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
-        case 4: launch_batch_code<uint32_t, uint8_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
-        default: launch_batch_code<uint32_t, uint16_t, uint16_t>(dE, item_mat, g0, g1, alpha, beta, st, cycles); break;
-        }
+    if (G > 0) {
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((G + 3) / 4, 8192));
+        if (cycles)
+            hipLaunchKernelGGL(k_batch_spmv<true>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat, G, alpha,
+                               beta, cycles);
+        else
+            hipLaunchKernelGGL(k_batch_spmv<false>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat, G,
+                               alpha, beta, cycles);
+        ABHSF_HIP(hipGetLastError());
     }
     if (nheavy > 0) {
         if (cycles)

@@ -66,14 +66,14 @@ struct BatchEntry {
 constexpr int kBatchCodes = 6;
 int batch_code(int bcol_bytes, int s);
 
-// All matrices' work items, one grid-stride launch per index-type combination
-// c over global items [code_begin[c], code_end[c]), then the heavy-strip
-// fix-up (dheavy: pairs {matrix, heavy index}).  item_mat[g]: matrix of
-// global item g.  FP64 storage only.  With cycles (device, [nmat], zeroed by
-// the caller) every wave adds its shader-clock cycles to its matrix's counter.
-void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, const int64_t *code_begin,
-                const int64_t *code_end, const uint32_t *dheavy, int64_t nheavy, double alpha, double beta,
-                hipStream_t st, unsigned long long *cycles = nullptr);
+// All matrices' work items [0, G) in one grid-stride launch (each member's
+// index-type combination, BatchEntry::code, selects its decode), then the
+// heavy-strip fix-up (dheavy: pairs {matrix, heavy index}).  item_mat[g]:
+// matrix of global item g.  FP64 storage only.  With cycles (device, [nmat],
+// zeroed by the caller) every wave adds its shader-clock cycles to its
+// matrix's counter.
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t G, const uint32_t *dheavy, int64_t nheavy,
+                double alpha, double beta, hipStream_t st, unsigned long long *cycles = nullptr);
 
 // Symmetric storage, second pass: Y[j] += scale * T_off^T * X[j] for the
 // stored triangle T without its diagonal (global atomics; FP64 values).
