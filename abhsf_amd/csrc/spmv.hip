@@ -410,6 +410,67 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
     }
 }
 
+// Dense group of a 32-row strip, FP64 values, coalesced: wave step q reads
+// positions q*128 + 2l, +1 of the block (1 KB contiguous); lane l always has
+// columns 2(l % 16), +1 and rows 4q + l/16 (q = 0..7), so its x pair is read
+// once per block and it keeps one accumulator per q over all the group's
+// blocks; the 16 lanes of a DPP row (one matrix row) are summed at the end.
+// (Lanes owning 16 consecutive positions made every load touch 64 lines.)
+template <typename BT>
+__device__ inline void dense32_r1(const DevMat &A, uint32_t b0, uint32_t b1, const double *__restrict__ x,
+                                  const Acc &acc, int lane)
+{
+    if (b0 >= b1) return;
+    const double *__restrict__ val = static_cast<const double *>(A.val[DENSE]);
+    const int64_t n = A.n;
+    const int cl = 2 * (lane & 15);
+    double a[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) a[q] = 0.0;
+#pragma unroll 1
+    for (uint32_t b = b0; b < b1; b++) {
+        const int64_t c0 = (int64_t)ld<BT>(A.bcol[DENSE], b) * 32 + cl;
+        double x0 = 0.0, x1 = 0.0;  // columns past n hold stored zeros; x is not read there
+        if (c0 + 1 < n) {
+            const double2 xv = *reinterpret_cast<const double2 *>(x + c0);
+            x0 = xv.x;
+            x1 = xv.y;
+        } else if (c0 < n) {
+            x0 = x[c0];
+        }
+        const double2 *__restrict__ vp = reinterpret_cast<const double2 *>(val + (uint64_t)b * 1024) + lane;
+#pragma unroll
+        for (int h = 0; h < 8; h += 4) {  // four loads in flight (register budget)
+            double2 v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] = vp[(h + q) * 64];
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[h + q] = __builtin_fma(v[q].y, x1, __builtin_fma(v[q].x, x0, a[h + q]));
+            __builtin_amdgcn_sched_barrier(0);  // keep the next four loads behind these
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const double r = group_sum_dpp<16>(a[q]);
+        if ((lane & 15) == 0) acc.hi[4 * q + (lane >> 4)] += r;
+    }
+}
+
+// Dense group of strip I: the coalesced path for FP64 32-row blocks, the
+// bitmap/dense walk otherwise.
+template <typename VT, typename BT, bool COMP>
+__device__ inline void dense_r1(const DevMat &A, uint32_t b0, uint32_t b1, const double *__restrict__ x,
+                                const Acc &acc, int lane)
+{
+    if constexpr (sizeof(VT) == 8 && !COMP) {
+        if (A.s == 32) {
+            dense32_r1<BT>(A, b0, b1, x, acc, lane);
+            return;
+        }
+    }
+    bmdn_r1<VT, BT, COMP>(A, DENSE, b0, b1, 0, x, acc, lane);
+}
+
 // Work items: items [0, n_light) are light items -- whole strips light[p] for
 // p in [pack[item], pack[item + 1]) (p = item without pack; strip p without
 // light) -- whose rows are written to y; items n_light + u are
@@ -483,7 +544,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
                 wave_sync();
                 bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x,
                                       at, lane);
-                bmdn_r1<VT, BT, COMP>(A, DENSE, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], 0, x, at, lane);
+                dense_r1<VT, BT, COMP>(A, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], x, at, lane);
                 wave_sync();
             }
             for (int r = lane; r < rows; r += kWave) {
@@ -514,7 +575,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
             wave_sync();
             bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x, acc,
                                   lane);
-            bmdn_r1<VT, BT, COMP>(A, DENSE, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], 0, x, acc, lane);
+            dense_r1<VT, BT, COMP>(A, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], x, acc, lane);
             wave_sync();
             for (int r = lane; r < s; r += kWave) {
                 const int64_t gr = I * s + r;
@@ -539,7 +600,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         else if (sc == BITMAP)
             bmdn_r1<VT, BT, COMP>(A, BITMAP, b0, b1, e0, x, acc, lane);
         else
-            bmdn_r1<VT, BT, COMP>(A, DENSE, b0, b1, e0, x, acc, lane);
+            dense_r1<VT, BT, COMP>(A, b0, b1, x, acc, lane);
         wave_sync();
         double *part = B.part + u * (size_t)s * (COMP ? 2 : 1);
         for (int r = lane; r < s; r += kWave) {
