@@ -410,26 +410,29 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
     }
 }
 
-// Dense group of a 32-row strip, FP64 values, coalesced: wave step q reads
-// positions q*128 + 2l, +1 of the block (1 KB contiguous); lane l always has
-// columns 2(l % 16), +1 and rows 4q + l/16 (q = 0..7), so its x pair is read
-// once per block and it keeps one accumulator per q over all the group's
-// blocks; the 16 lanes of a DPP row (one matrix row) are summed at the end.
-// (Lanes owning 16 consecutive positions made every load touch 64 lines.)
-template <typename BT>
-__device__ inline void dense32_r1(const DevMat &A, uint32_t b0, uint32_t b1, const double *__restrict__ x,
-                                  const Acc &acc, int lane)
+// Dense group of an S-row strip (S = 16 or 32), FP64 values, coalesced: wave
+// step q reads positions q*128 + 2l, +1 of the block (1 KB contiguous); lane
+// l always has columns 2(l % L), +1 (L = S/2 lanes per row) and rows
+// q*(128/S) + l/L, so its x pair is read once per block and it keeps one
+// accumulator per step over all the group's blocks; the L lanes of a row are
+// summed with DPP at the end.  (Lanes owning S*S/64 consecutive positions
+// made a load touch up to 64 lines: S = 32 streamed 3.8 TB/s, now 5.6.)
+template <typename BT, int S>
+__device__ inline void dense_coal_r1(const DevMat &A, uint32_t b0, uint32_t b1, const double *__restrict__ x,
+                                     const Acc &acc, int lane)
 {
+    static_assert(S == 16 || S == 32, "S x S blocks of whole 128-position steps");
+    constexpr int L = S / 2, NQ = S * S / 128, RQ = 128 / S;
     if (b0 >= b1) return;
     const double *__restrict__ val = static_cast<const double *>(A.val[DENSE]);
     const int64_t n = A.n;
-    const int cl = 2 * (lane & 15);
-    double a[8];
+    const int cl = 2 * (lane % L);
+    double a[NQ];
 #pragma unroll
-    for (int q = 0; q < 8; q++) a[q] = 0.0;
+    for (int q = 0; q < NQ; q++) a[q] = 0.0;
 #pragma unroll 1
     for (uint32_t b = b0; b < b1; b++) {
-        const int64_t c0 = (int64_t)ld<BT>(A.bcol[DENSE], b) * 32 + cl;
+        const int64_t c0 = (int64_t)ld<BT>(A.bcol[DENSE], b) * S + cl;
         double x0 = 0.0, x1 = 0.0;  // columns past n hold stored zeros; x is not read there
         if (c0 + 1 < n) {
             const double2 xv = *reinterpret_cast<const double2 *>(x + c0);
@@ -438,33 +441,38 @@ __device__ inline void dense32_r1(const DevMat &A, uint32_t b0, uint32_t b1, con
         } else if (c0 < n) {
             x0 = x[c0];
         }
-        const double2 *__restrict__ vp = reinterpret_cast<const double2 *>(val + (uint64_t)b * 1024) + lane;
+        const double2 *__restrict__ vp = reinterpret_cast<const double2 *>(val + (uint64_t)b * (S * S)) + lane;
 #pragma unroll
-        for (int h = 0; h < 8; h += 4) {  // four loads in flight (register budget)
-            double2 v[4];
+        for (int h = 0; h < NQ; h += 4) {  // at most four loads in flight (register budget)
+            constexpr int H = NQ < 4 ? NQ : 4;
+            double2 v[H];
 #pragma unroll
-            for (int q = 0; q < 4; q++) v[q] = vp[(h + q) * 64];
+            for (int q = 0; q < H; q++) v[q] = vp[(h + q) * 64];
 #pragma unroll
-            for (int q = 0; q < 4; q++) a[h + q] = __builtin_fma(v[q].y, x1, __builtin_fma(v[q].x, x0, a[h + q]));
-            __builtin_amdgcn_sched_barrier(0);  // keep the next four loads behind these
+            for (int q = 0; q < H; q++) a[h + q] = __builtin_fma(v[q].y, x1, __builtin_fma(v[q].x, x0, a[h + q]));
+            __builtin_amdgcn_sched_barrier(0);  // keep the next loads behind these
         }
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const double r = group_sum_dpp<16>(a[q]);
-        if ((lane & 15) == 0) acc.hi[4 * q + (lane >> 4)] += r;
+    for (int q = 0; q < NQ; q++) {
+        const double r = group_sum_dpp<L>(a[q]);
+        if (lane % L == 0) acc.hi[q * RQ + lane / L] += r;
     }
 }
 
-// Dense group of strip I: the coalesced path for FP64 32-row blocks, the
-// bitmap/dense walk otherwise.
+// Dense group of strip I: the coalesced path for FP64 16- and 32-row blocks,
+// the bitmap/dense walk otherwise.
 template <typename VT, typename BT, bool COMP>
 __device__ inline void dense_r1(const DevMat &A, uint32_t b0, uint32_t b1, const double *__restrict__ x,
                                 const Acc &acc, int lane)
 {
     if constexpr (sizeof(VT) == 8 && !COMP) {
         if (A.s == 32) {
-            dense32_r1<BT>(A, b0, b1, x, acc, lane);
+            dense_coal_r1<BT, 32>(A, b0, b1, x, acc, lane);
+            return;
+        }
+        if (A.s == 16) {
+            dense_coal_r1<BT, 16>(A, b0, b1, x, acc, lane);
             return;
         }
     }
