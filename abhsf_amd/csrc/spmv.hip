@@ -461,16 +461,20 @@ __device__ inline void dense_coal_r1(const DevMat &A, uint32_t b0, uint32_t b1, 
 }
 
 // Dense group of strip I: the coalesced path for FP64 16- and 32-row blocks,
-// the bitmap/dense walk otherwise.
-template <typename VT, typename BT, bool COMP>
+// the bitmap/dense walk otherwise.  The index types tell the block-size class
+// at compile time (u8 indices with u16 row pointers: s = 16; u16 indices:
+// s >= 32), so each kernel carries only the path it can take.
+template <typename VT, typename BT, typename IT, typename RT, bool COMP>
 __device__ inline void dense_r1(const DevMat &A, uint32_t b0, uint32_t b1, const double *__restrict__ x,
                                 const Acc &acc, int lane)
 {
-    if constexpr (sizeof(VT) == 8 && !COMP) {
+    if constexpr (sizeof(VT) == 8 && !COMP && sizeof(IT) == 2) {
         if (A.s == 32) {
             dense_coal_r1<BT, 32>(A, b0, b1, x, acc, lane);
             return;
         }
+    }
+    if constexpr (sizeof(VT) == 8 && !COMP && sizeof(IT) == 1 && sizeof(RT) == 2) {
         if (A.s == 16) {
             dense_coal_r1<BT, 16>(A, b0, b1, x, acc, lane);
             return;
@@ -501,7 +505,10 @@ __device__ inline int64_t xcd_block_id()
 
 // One work item of one matrix (see Balance): the wave accumulates in acc_buf
 // (s doubles, 2s when compensated) and writes y rows or a unit's partials.
-template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS = 2, int CH = 4>
+// TL: 0 the matrix is not tiled, 1 it is, 2 either (read from B.tiled); the
+// single-matrix kernels are compiled for 0 or 1 so that each carries only one
+// of the light-item paths (scalar-register pressure).
+template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS = 2, int CH = 4, int TL = 2>
 __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item, const double *__restrict__ x,
                                  double *__restrict__ y, double alpha, double beta, double *acc_buf,
                                  uint32_t *s_end, uint32_t *s_bc, uint32_t *s_st, uint32_t *s_to, int lane)
@@ -522,7 +529,8 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
             p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item]);
             p1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)B.pack[item + 1]);
         }
-        if (B.tiled) {
+        const bool tiled = TL == 1 || (TL == 2 && B.tiled);
+        if (tiled) {
             // the item's strips light[p0 .. p1): one pass over their COO
             // blocks (rows (position << k) + lrow), then each strip's CSR /
             // bitmap / dense groups, then all rows
@@ -552,7 +560,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
                 wave_sync();
                 bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x,
                                       at, lane);
-                dense_r1<VT, BT, COMP>(A, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], x, at, lane);
+                dense_r1<VT, BT, IT, RT, COMP>(A, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], x, at, lane);
                 wave_sync();
             }
             for (int r = lane; r < rows; r += kWave) {
@@ -583,7 +591,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
             wave_sync();
             bmdn_r1<VT, BT, COMP>(A, BITMAP, A.sptr[BITMAP][I], A.sptr[BITMAP][I + 1], A.eptr[BITMAP][I], x, acc,
                                   lane);
-            dense_r1<VT, BT, COMP>(A, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], x, acc, lane);
+            dense_r1<VT, BT, IT, RT, COMP>(A, A.sptr[DENSE][I], A.sptr[DENSE][I + 1], x, acc, lane);
             wave_sync();
             for (int r = lane; r < s; r += kWave) {
                 const int64_t gr = I * s + r;
@@ -608,7 +616,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         else if (sc == BITMAP)
             bmdn_r1<VT, BT, COMP>(A, BITMAP, b0, b1, e0, x, acc, lane);
         else
-            dense_r1<VT, BT, COMP>(A, b0, b1, x, acc, lane);
+            dense_r1<VT, BT, IT, RT, COMP>(A, b0, b1, x, acc, lane);
         wave_sync();
         double *part = B.part + u * (size_t)s * (COMP ? 2 : 1);
         for (int r = lane; r < s; r += kWave) {
@@ -624,7 +632,7 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
 // light) -- whose rows are written to y; items n_light + u are
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
-template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS, int CH>
+template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS, int CH, bool TILED>
 __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__restrict__ x, double *__restrict__ y,
                                                     double alpha, double beta, Balance B)
 {
@@ -637,17 +645,17 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     __shared__ uint32_t s_st[kWaves][kWave];
     __shared__ uint32_t s_to[kWaves][kWave];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    int64_t item;
-    if (B.order) {  // column panels: the slot table already places items on XCDs
-        const uint32_t o = B.order[(int64_t)blockIdx.x * kWaves + wave];
-        if (o == kNoItem) return;
-        item = (int64_t)__builtin_amdgcn_readfirstlane((int)o);
-    } else {
-        item = xcd_block_id() * kWaves + wave;
+    int64_t item = xcd_block_id() * kWaves + wave;
+    if constexpr (!TILED) {
+        if (B.order) {  // column panels: the slot table already places items on XCDs
+            const uint32_t o = B.order[(int64_t)blockIdx.x * kWaves + wave];
+            if (o == kNoItem) return;
+            item = (int64_t)__builtin_amdgcn_readfirstlane((int)o);
+        }
     }
     if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
-    spmv_item<VT, BT, IT, RT, COMP, ZS, CH>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave], s_bc[wave],
-                                            s_st[wave], s_to[wave], lane);
+    spmv_item<VT, BT, IT, RT, COMP, ZS, CH, TILED ? 1 : 0>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave],
+                                                           s_bc[wave], s_st[wave], s_to[wave], lane);
 }
 
 // ===========================================================================
@@ -1296,12 +1304,19 @@ void launch_spmv(const DevMat &A, const Balance &B, bool coo_single, const doubl
     // COO blocks of (almost) one entry: phase A takes one entry from each of
     // 8 x 64 blocks per step (more gathers in flight, no unused entry slot);
     // otherwise two entries from each of 4 x 64 blocks
-    if (coo_single)
-        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 8>), dim3((unsigned)nblk), dim3(256), 0, st, A, x,
-                           y, alpha, beta, b);
+    // (tiled matrices have their own kernels: each carries one light-item path)
+    if (coo_single && b.tiled)
+        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 8, true>), dim3((unsigned)nblk), dim3(256), 0, st,
+                           A, x, y, alpha, beta, b);
+    else if (coo_single)
+        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 8, false>), dim3((unsigned)nblk), dim3(256), 0, st,
+                           A, x, y, alpha, beta, b);
+    else if (b.tiled)
+        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 2, 4, true>), dim3((unsigned)nblk), dim3(256), 0, st,
+                           A, x, y, alpha, beta, b);
     else
-        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 2, 4>), dim3((unsigned)nblk), dim3(256), 0, st, A, x,
-                           y, alpha, beta, b);
+        hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 2, 4, false>), dim3((unsigned)nblk), dim3(256), 0, st,
+                           A, x, y, alpha, beta, b);
     ABHSF_HIP(hipGetLastError());
     if (b.n_heavy) {
         hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)b.n_heavy), dim3(kFixThreads), 0, st, A.s,
