@@ -7,6 +7,7 @@
 //
 // usage: spmv_unified <dense|diagonal|single-column|random-column|
 //                      single-row-column|single-row|file.mtx> [n] [block_size]
+//                      [strip|balanced|panel|tiled]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -73,7 +74,8 @@ static void read_mtx(const std::string &fn, elements_t &el, matrix_properties &p
 int main(int argc, char *argv[])
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <family|file.mtx> [n] [block_size]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <family|file.mtx> [n] [block_size] [strip|balanced|panel|tiled]\n",
+                     argv[0]);
         return 2;
     }
     try {
@@ -82,6 +84,15 @@ int main(int argc, char *argv[])
         const std::string kind = argv[1];
         const uintmax_t n = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 0;
         const int s = argc > 3 ? std::atoi(argv[3]) : 64;
+        int variant = ABHSF_VARIANT_AUTO;
+        if (argc > 4) {
+            const std::string v = argv[4];
+            if (v == "strip") variant = ABHSF_VARIANT_STRIP;
+            else if (v == "balanced") variant = ABHSF_VARIANT_BALANCED;
+            else if (v == "panel") variant = ABHSF_VARIANT_PANEL;
+            else if (v == "tiled") variant = ABHSF_VARIANT_TILED;
+            else throw std::runtime_error("unknown variant " + v);
+        }
         props.type = matrix_type_t::REAL;
         props.symmetry = matrix_symmetry_t::UNSYMMETRIC;
         // matrix families of spmv/unified/spmv.cpp:1078-1156 (sizes default smaller)
@@ -142,7 +153,7 @@ int main(int argc, char *argv[])
         std::cout << "Expected result = " << std::sqrt(expected) << std::endl;
 
         const double n_mflops = (double)(nnz_all * 2 * n_iters) / 1.0e6;
-        abhsf_matrix A(s);
+        abhsf_matrix A(s, ABHSF_POLICY_PREPROC, ABHSF_F64, variant);
         A.from_elements(elements, props);
         const abhsf_stats st = A.stats();
         std::cout << "Block size: " << s << ", blocks COO/CSR/bitmap/dense: " << st.blocks[0] << "/" << st.blocks[1]

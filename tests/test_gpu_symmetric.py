@@ -137,7 +137,8 @@ def test_symmetric_rejects_both_triangles_and_skew_diagonal(gpu):
         AbhsfBatch([M])
 
 
-@pytest.mark.parametrize("case", ["dense", "random-column", "mtx_general", "mtx_symmetric", "mtx_skew"])
+@pytest.mark.parametrize("case", ["dense", "random-column", "mtx_general", "mtx_symmetric", "mtx_skew",
+                                  "random-column:tiled", "mtx_general:tiled"])
 def test_cpp_harness(gpu, case, tmp_path):
     """The C++ harness (abhsf_amd/csrc/spmv_unified.cpp, the reference's
     spmv/unified driver on this library): x = 0.5, 5 + 20 accumulating calls,
@@ -148,6 +149,7 @@ def test_cpp_harness(gpu, case, tmp_path):
     from conftest import ROOT
     exe = os.path.join(ROOT, "abhsf_amd", "spmv_unified")
     assert os.path.exists(exe), "build with __graft_entry__.build()"
+    case, _, variant = case.partition(":")
     if case.startswith("mtx"):
         kind = case[4:]
         A = synth.laplacian2d(30) if kind != "general" else synth.random_sparse(500, 500, 0.02, 7)
@@ -162,7 +164,9 @@ def test_cpp_harness(gpu, case, tmp_path):
         args = [exe, str(path), "0", "32"]
     else:
         args = [exe, case, "3000", "64"]
-    print("case harness", case, flush=True)
+    if variant:
+        args.append(variant)
+    print("case harness", case, variant, flush=True)
     out = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     exp = float(re.search(r"Expected result = ([0-9.eE+-]+)", out.stdout).group(1))
