@@ -646,12 +646,10 @@ __global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__re
     __shared__ uint32_t s_to[kWaves][kWave];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     int64_t item = xcd_block_id() * kWaves + wave;
-    if constexpr (!TILED) {
-        if (B.order) {  // column panels: the slot table already places items on XCDs
-            const uint32_t o = B.order[(int64_t)blockIdx.x * kWaves + wave];
-            if (o == kNoItem) return;
-            item = (int64_t)__builtin_amdgcn_readfirstlane((int)o);
-        }
+    if (B.order) {  // a slot table (column panels: items per XCD; tiles: launch order)
+        const uint32_t o = B.order[(int64_t)blockIdx.x * kWaves + wave];
+        if (o == kNoItem) return;
+        item = (int64_t)__builtin_amdgcn_readfirstlane((int)o);
     }
     if (item >= B.n_light + B.n_units) return;  // whole wave leaves; no workgroup barriers below
     spmv_item<VT, BT, IT, RT, COMP, ZS, CH, TILED ? 1 : 0>(A, B, item, x, y, alpha, beta, s_acc[wave], s_end[wave],
