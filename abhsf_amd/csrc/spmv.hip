@@ -905,7 +905,14 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
 // L threads are combined in a fixed order, so the sum does not depend on
 // timing.  Partials of unit u: part[u * s * (COMP ? 2 : 1) + r] (+ s: the
 // compensation terms).
-constexpr int kFixThreads = 1024;
+// (512 threads: R-MAT scale 22 at s = 256, ~9 units per heavy strip, spent
+// 74 us per SpMV in this pass with 1024 threads and 26 us less with 512; with
+// 256 one thread per row had too few loads in flight)
+#ifndef ABHSF_FIX_THREADS
+#define ABHSF_FIX_THREADS 512
+#endif
+constexpr int kFixThreads = ABHSF_FIX_THREADS;
+static_assert(kFixThreads >= 256, "at least one thread per row of a 256-row strip");
 
 template <bool COMP>
 __device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
