@@ -95,7 +95,10 @@ void reset_storage(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz)
 
 // Work decomposition for the single-RHS kernel: units of about 2048 entries
 // (at least 8 rows' worth), heavy strips above twice that.
-int64_t balance_unit(const abhsf::Storage &S) { return std::max<int64_t>(2048, 8 * (int64_t)S.s); }
+#ifndef ABHSF_UNIT_ENTRIES
+#define ABHSF_UNIT_ENTRIES 2048
+#endif
+int64_t balance_unit(const abhsf::Storage &S) { return std::max<int64_t>(ABHSF_UNIT_ENTRIES, 8 * (int64_t)S.s); }
 
 void finish_build(abhsf_handle *h)
 {
