@@ -466,8 +466,8 @@ def main():
         print(json.dumps(summarize_pmc(a[0], a[1], a[2], a[3] if len(a) == 4 else None, args.pmc_command)))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the SpMV shards by block-row strips; only the single-GPU kernel is built)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the SpMV would shard by "
+              "block-row strips; only the single-GPU kernel is implemented)", file=sys.stderr)
         sys.exit(2)
     from abhsf_amd import hip
     run = {"block_mix": run_block_mix, "rmat": run_rmat, "spmm": run_spmm, "batch": run_batch}[args.workload]
