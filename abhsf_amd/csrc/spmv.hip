@@ -213,46 +213,54 @@ __device__ inline void coo_r1(const DevMat &A, uint32_t b0, uint32_t b1, uint32_
         }
         auto zof = [&](int c) { return TILE ? zt[c] & 0x1FFFFu : zt[c]; };
         auto tzof = [&](int c) { return TILE ? (zt[c] >> 17) << (2 * k) : 0u; };
+        const uint32_t e_step = e;
 #pragma unroll
         for (int c = 0; c < CH; c++) {
             const uint32_t incl = wave_incl_scan_dpp(zof(c));
             st[c] = e + incl - zof(c);
             e += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
         }
-        uint32_t id[CH][ZS];
-        double v[CH][ZS], xv[CH][ZS];
+        // blocks of 16 entries or more on average (a uniform decision): phase
+        // A would keep a few lanes busy for a whole memory round trip, so
+        // phase B takes every entry
+        const uint32_t nbs = min<uint32_t>(b1 - cb, (uint32_t)(kWave * CH));
+        const uint32_t za = (e - e_step >= 16u * nbs) ? 0u : (uint32_t)ZS;
+        if (za) {
+            uint32_t id[CH][ZS];
+            double v[CH][ZS], xv[CH][ZS];
 #pragma unroll
-        for (int c = 0; c < CH; c++)
+            for (int c = 0; c < CH; c++)
 #pragma unroll
-            for (int i = 0; i < ZS; i++) {
-                const bool h = (uint32_t)i < zof(c);
-                id[c][i] = h ? (uint32_t)idx[st[c] + i] + tzof(c) : 0u;
-                v[c][i] = h ? (double)val[st[c] + i] : 0.0;
-            }
-#pragma unroll
-        for (int c = 0; c < CH; c++)
-#pragma unroll
-            for (int i = 0; i < ZS; i++)
-                xv[c][i] = ((uint32_t)i < zof(c)) ? x[(int64_t)bc[c] * s + (id[c][i] & smask)] : 0.0;
-#pragma unroll
-        for (int c = 0; c < CH; c++)
-#pragma unroll
-            for (int i = 0; i < ZS; i++)
-                if ((uint32_t)i < zof(c)) {
-                    double pe;
-                    const double p = prod<COMP>(v[c][i], xv[c][i], pe);
-                    acc_atomic<COMP>(acc, id[c][i] >> k, p, pe);
+                for (int i = 0; i < ZS; i++) {
+                    const bool h = (uint32_t)i < zof(c);
+                    id[c][i] = h ? (uint32_t)idx[st[c] + i] + tzof(c) : 0u;
+                    v[c][i] = h ? (double)val[st[c] + i] : 0.0;
                 }
-        // phase B: entries beyond the first ZS of a block, one per lane
+#pragma unroll
+            for (int c = 0; c < CH; c++)
+#pragma unroll
+                for (int i = 0; i < ZS; i++)
+                    xv[c][i] = ((uint32_t)i < zof(c)) ? x[(int64_t)bc[c] * s + (id[c][i] & smask)] : 0.0;
+#pragma unroll
+            for (int c = 0; c < CH; c++)
+#pragma unroll
+                for (int i = 0; i < ZS; i++)
+                    if ((uint32_t)i < zof(c)) {
+                        double pe;
+                        const double p = prod<COMP>(v[c][i], xv[c][i], pe);
+                        acc_atomic<COMP>(acc, id[c][i] >> k, p, pe);
+                    }
+        }
+        // phase B: entries beyond the first za of a block, one per lane
 #pragma unroll 1
         for (int c = 0; c < CH; c++) {
-            const uint32_t rem = zof(c) > (uint32_t)ZS ? zof(c) - (uint32_t)ZS : 0u;
+            const uint32_t rem = zof(c) > za ? zof(c) - za : 0u;
             if (__ballot(rem != 0) == 0) continue;
             const uint32_t rincl = wave_incl_scan_dpp(rem);
             const uint32_t RT = (uint32_t)__builtin_amdgcn_readlane((int)rincl, kWave - 1);
             s_end[lane] = rincl;
             s_bc[lane] = bc[c];
-            s_st[lane] = st[c] + (uint32_t)ZS;
+            s_st[lane] = st[c] + za;
             if constexpr (TILE) s_to[lane] = tzof(c);
             wave_sync();
             const uint32_t cs = cb + (uint32_t)(c * kWave);
