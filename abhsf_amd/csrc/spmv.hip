@@ -909,18 +909,23 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
 // Rows of one heavy strip I: y = alpha * (sum of the partial rows of units
 // [u0, u0 + nu)) + beta * y, by a workgroup of kFixThreads threads.  The
 // L = kFixThreads / s threads of row r take units l, l + L, l + 2L, ... in
-// four interleaved chains (several loads in flight); the chains and then the
-// L threads are combined in a fixed order, so the sum does not depend on
-// timing.  Partials of unit u: part[u * s * (COMP ? 2 : 1) + r] (+ s: the
-// compensation terms).
-// (512 threads: R-MAT scale 22 at s = 256, ~9 units per heavy strip, spent
-// 74 us per SpMV in this pass with 1024 threads and 26 us less with 512; with
-// 256 one thread per row had too few loads in flight)
+// kFixChains interleaved chains (that many loads in flight per thread); the
+// chains and then the L threads are combined in a fixed order, so the sum
+// does not depend on timing.  Partials of unit u: part[u * s * (COMP ? 2 : 1)
+// + r] (+ s: the compensation terms).
+// (R-MAT scale 22 at s = 256: the heaviest strip holds 584 units and sets the
+// pass's length.  SpMV 0.545 ms with 512 threads x 4 chains, 0.532 ms with
+// 256 x 16; 256 x 8 and 512 x 16 are in between, 32 chains spill: 0.61+ ms.
+// 1024 threads x 4 chains was slower than 512 x 4.)
 #ifndef ABHSF_FIX_THREADS
-#define ABHSF_FIX_THREADS 512
+#define ABHSF_FIX_THREADS 256
 #endif
 constexpr int kFixThreads = ABHSF_FIX_THREADS;
 static_assert(kFixThreads >= 256, "at least one thread per row of a 256-row strip");
+#ifndef ABHSF_FIX_CHAINS
+#define ABHSF_FIX_CHAINS 16
+#endif
+constexpr int kFixChains = ABHSF_FIX_CHAINS;
 
 template <bool COMP>
 __device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
@@ -943,11 +948,13 @@ __device__ inline void fixup_strip(int s, int64_t m, int64_t I, const double *__
     const int r = (int)(threadIdx.x / L);
     const uint32_t l = threadIdx.x % L;
     const size_t stride = (size_t)s * (COMP ? 2 : 1);
-    double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
-    uint32_t u = l;
-    for (; u + 3u * L < nu; u += 4u * L) {
+    double hi[kFixChains], lo[kFixChains];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kFixChains; j++) hi[j] = lo[j] = 0.0;
+    uint32_t u = l;
+    for (; u + (uint32_t)(kFixChains - 1) * L < nu; u += (uint32_t)kFixChains * L) {
+#pragma unroll
+        for (int j = 0; j < kFixChains; j++) {
             const double *p = part + (size_t)(u0 + u + (uint32_t)j * L) * stride;
             fix_add<COMP>(hi[j], lo[j], p[r], COMP ? p[s + r] : 0.0);
         }
@@ -957,7 +964,7 @@ __device__ inline void fixup_strip(int s, int64_t m, int64_t I, const double *__
         fix_add<COMP>(hi[0], lo[0], p[r], COMP ? p[s + r] : 0.0);
     }
 #pragma unroll
-    for (int j = 1; j < 4; j++) fix_add<COMP>(hi[0], lo[0], hi[j], lo[j]);
+    for (int j = 1; j < kFixChains; j++) fix_add<COMP>(hi[0], lo[0], hi[j], lo[j]);
     sh[threadIdx.x] = hi[0];
     if constexpr (COMP) sh[kFixThreads + threadIdx.x] = lo[0];
     __syncthreads();
