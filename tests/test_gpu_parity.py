@@ -6,6 +6,8 @@ SpMV of the same inputs (TOL below).  FP32 value storage: error against the
 FP64 oracle on the FP32-rounded values <= 1e-12 (compensated accumulation),
 and the error against the FP64 values is reported.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -272,6 +274,45 @@ def test_heavy_strip_variants(gpu, variant, precision, s):
     vals = A.val if precision == "f64" else A.val.astype(np.float32).astype(np.float64)
     ref = pyoracle.csr_spmv(A.rowptr, A.col, vals, x, 2.0, 0.5, y0)
     assert relerr(dy.numpy(), ref) <= TOL
+
+
+@functools.lru_cache(maxsize=1)
+def two_full_rows(n=1 << 21, seed=0):
+    """Rows 0 and 1 hold every column (2n entries: the strip that contains
+    them is cut into ~2n/2048 units), the other rows up to 3 random entries;
+    integer values in [-8, 8] \\ {0}."""
+    rng = np.random.default_rng(seed)
+    r = np.repeat(np.arange(2, n, dtype=np.int64), rng.integers(0, 4, n - 2))
+    key = np.unique(r * n + rng.integers(0, n, r.size))  # sorted, duplicates merged
+    rows = np.concatenate([np.zeros(n, np.int64), np.ones(n, np.int64), key // n])
+    col = np.concatenate([np.arange(n), np.arange(n), key % n]).astype(np.int32)
+    rowptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    val = (rng.integers(1, 9, col.size) * rng.choice([-1.0, 1.0], col.size)).astype(np.float64)
+    return synth.Csr(n, n, rowptr, col, val, "two_full_rows")
+
+
+@pytest.mark.parametrize("variant,precision", [("balanced", "f64"), ("tiled", "f64"), ("balanced", "f32")])
+@pytest.mark.parametrize("s", [2, 8, 256])
+def test_fixup_many_units_per_strip(gpu, variant, precision, s):
+    """One strip of 2048+ units: the fix-up's threads of a row (256/s of
+    them) each walk many units in their interleaved chains and then the
+    remainder.  Integer data keeps every sum exact, so y must equal the
+    oracle bit for bit whatever the summation order."""
+    A = two_full_rows()
+    print(f"case two_full_rows s={s} {variant} {precision}", flush=True)
+    M = AbhsfMatrix(block_size=s, precision=precision, variant=variant)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    x = np.random.default_rng(1).integers(-8, 9, A.n).astype(np.float64)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.zeros(A.m)
+    M.spmv_async(1.0, dx, 0.0, dy)
+    M.synchronize()
+    ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)
+    assert np.array_equal(dy.numpy(), ref)
+    y0 = np.random.default_rng(2).uniform(-1, 1, A.m)
+    dy = DeviceArray.from_numpy(y0)
+    M.spmv_async(-0.5, dx, 2.0, dy)
+    M.synchronize()
+    assert relerr(dy.numpy(), pyoracle.csr_spmv(A.rowptr, A.col, A.val, x, -0.5, 2.0, y0)) <= TOL
 
 
 PANEL_MATS = {
