@@ -1056,6 +1056,17 @@ __device__ inline void dense_block_mfma(const VT *__restrict__ D, int s, int64_t
     }
 }
 
+// Rows per lane and step in the SpMM kernel's CSR group (their loads are in
+// flight together).  A block takes s * R / (64 * kSpmmRows) steps, each a
+// chain of dependent loads (row pointers, columns, X), so with one row per
+// lane the time grew with R.  27-point stencil 160^3, s = 64, FP64: k = 8
+// 1.542 -> 1.388 ms, k = 32 6.51 -> 5.86 ms with 2 rows; 4 and 8 rows were
+// slower than 1 (k = 8: 1.70 and 2.42 ms).
+#ifndef ABHSF_SPMM_ROWS
+#define ABHSF_SPMM_ROWS 2
+#endif
+constexpr int kSpmmRows = ABHSF_SPMM_ROWS;
+
 template <typename VT, typename BT, typename IT, typename RT, int R, bool COMP>
 __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__restrict__ X, int64_t ldx,
                                                     double *__restrict__ Y, int64_t ldy, int nrhs,
@@ -1153,25 +1164,55 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
             const int64_t c0 = (int64_t)ld<BT>(A.bcol[CSR], b) * s;
             const RT *rp = rowptr + (uint64_t)b * (uint64_t)(s + 1);
 #pragma unroll 1
-            for (int r0 = 0; r0 < s; r0 += E) {
-                const int r = r0 + slot;
-                if (r < s && jok) {
-                    const uint32_t q0 = rp[r], q1 = rp[r + 1];
-                    double hi = 0.0, lo = 0.0;
-                    for (uint32_t qq = q0; qq < q1; qq++)
-                        acc_add<COMP>(hi, lo, (double)val[e + qq], X[(c0 + A.csr_col[e + qq]) * ldx + j]);
-                    if (q1 > q0) {
-                        const uint32_t ai = (uint32_t)r * R + j;
+            for (int r0 = 0; r0 < s; r0 += E * kSpmmRows) {
+                // kSpmmRows rows per lane (r0 + u*E + slot), walked together so
+                // that their loads are in flight at once; each row's entries
+                // are added in order
+                uint32_t qb[kSpmmRows], ql[kSpmmRows], len = 0;
+#pragma unroll
+                for (int u = 0; u < kSpmmRows; u++) {
+                    const int r = r0 + u * E + slot;
+                    const bool ok = r < s && jok;
+                    qb[u] = ok ? (uint32_t)rp[r] : 0u;
+                    ql[u] = ok ? (uint32_t)rp[r + 1] - qb[u] : 0u;
+                    len = max(len, ql[u]);
+                }
+                double hi[kSpmmRows], lo[kSpmmRows];
+#pragma unroll
+                for (int u = 0; u < kSpmmRows; u++) hi[u] = lo[u] = 0.0;
+#pragma unroll 1
+                for (uint32_t t = 0; t < len; t++) {
+                    // all rows' column and value loads, then their x loads,
+                    // then the sums
+                    uint32_t cc[kSpmmRows];
+                    double v[kSpmmRows], xv[kSpmmRows];
+#pragma unroll
+                    for (int u = 0; u < kSpmmRows; u++) {
+                        const bool h = t < ql[u];
+                        const uint64_t q = e + qb[u] + t;
+                        cc[u] = h ? (uint32_t)A.csr_col[q] : 0u;
+                        v[u] = h ? (double)val[q] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kSpmmRows; u++)
+                        xv[u] = (t < ql[u]) ? X[(c0 + cc[u]) * ldx + j] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < kSpmmRows; u++)
+                        if (t < ql[u]) acc_add<COMP>(hi[u], lo[u], v[u], xv[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kSpmmRows; u++)
+                    if (ql[u]) {
+                        const uint32_t ai = (uint32_t)(r0 + u * E + slot) * R + j;
                         if constexpr (COMP) {
                             double sum, err;
-                            two_sum(acc.hi[ai], hi, sum, err);
+                            two_sum(acc.hi[ai], hi[u], sum, err);
                             acc.hi[ai] = sum;
-                            acc.lo[ai] += err + lo;
+                            acc.lo[ai] += err + lo[u];
                         } else {
-                            acc.hi[ai] += hi;
+                            acc.hi[ai] += hi[u];
                         }
                     }
-                }
             }
             wave_sync();
             e += (uint64_t)A.z1[CSR][b] + 1u;
