@@ -617,8 +617,11 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
         // constant scheme arguments: the descriptor's per-scheme arrays are
         // then indexed statically (a runtime index would force a local copy
         // of the descriptor, as in the batched kernel, into scratch memory)
+        // the tiled kernel's units take one entry per block in phase A (its
+        // tiles keep ZS): R-MAT scale 22 s = 64 / 128 / 256 0.623 / 0.590 /
+        // 0.536 -> 0.611 / 0.584 / 0.528 ms, same registers
         if (sc == COO)
-            coo_r1<VT, BT, IT, COMP, ZS, CH>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, s_to, lane);
+            coo_r1<VT, BT, IT, COMP, (TL == 1 ? 1 : ZS), CH>(A, b0, b1, e0, x, acc, s_end, s_bc, s_st, s_to, lane);
         else if (sc == CSR)
             csr_rows_r1<VT, BT, RT, COMP>(A, b0, b1, e0, x, acc, s_st, s_bc, lane);
         else if (sc == BITMAP)
