@@ -10,9 +10,10 @@
  *   orc_choose         closed-form crossover points solved by hand; App. B
  *                      footprints of preprocessing/test.cpp (test.mtx s=2..1024,
  *                      256^2 Laplacian s=8..256).
- *   orc_smf_preproc    the same known answers (preprocessing/test.cpp run by
- *                      the survey; its AQsort dependency is absent here, so
- *                      test.cpp itself cannot be rebuilt).
+ *   orc_smf_preproc    the same known answers (output of preprocessing/test.cpp
+ *                      recorded in tests/golden/known_answers.json; its AQsort
+ *                      dependency is absent here, so test.cpp itself cannot be
+ *                      rebuilt).
  *   orc_block_hist     oracle/_ref/block_stats (the reference's own binary,
  *                      compiled from block_statistics/block_stats.cpp) on
  *                      test.mtx and seeded random matrices; sum z*count = nnz.

@@ -80,8 +80,9 @@ def test_conversion_bit_exact_and_spmv(gpu, name, s):
 
 
 def test_config1_laplacian256_s64(gpu):
-    """256^2 5-point Laplacian, s=64, x ~ U[-1,1] seed 0,
-    y = A x checked against CSR FP64; footprint = the reference's 3,695,344 bits."""
+    """The reference's CPU-path case: 256^2 5-point Laplacian, s=64, x ~ U[-1,1]
+    seed 0, y = A x checked against CSR FP64; footprint = the reference's
+    3,695,344 bits."""
     A = synth.laplacian2d(256)
     assert A.nnz == 326656
     M = build(A, 64)

@@ -6,8 +6,9 @@ transposed operand fails at least one):
                 test.cpp:88-98), including powers of two and their neighbours.
   choose        closed-form crossovers solved by hand for s = 8, 64, 256.
   smf_preproc   golden known answers of preprocessing/test.cpp (test.mtx at
-                s = 2..1024; 256^2 Laplacian at s = 64), plus the survey's
-                restatement at the other s.
+                s = 2..1024; 256^2 Laplacian at s = 64), the Laplacian at the
+                other s from an independent restatement of test.cpp:55-77, and
+                a count by hand at s = 1024.
   block_hist    oracle/_ref/block_stats (the reference's binary) on test.mtx and
                 seeded random matrices; golden bstats; sum z*count = nnz.
   mmf_row       oracle/_ref/mmf (the reference's binary); golden rows.
