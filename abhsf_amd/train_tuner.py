@@ -1,4 +1,4 @@
-"""Train the learned tuner on the GPU and judge it against exhaustive search.
+"""Train the learned tuner on the GPU and evaluate it against exhaustive search.
 
   python -m abhsf_amd.train_tuner --out DIR [--mode latency|batch] [--steps N --lr R --members K]
 

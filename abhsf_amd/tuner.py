@@ -7,7 +7,7 @@ candidate with the smallest prediction.  Two models share the features: one
 trained on single-launch latency (MODEL_PATH), one on per-matrix throughput
 inside a batched launch (BATCH_MODEL_PATH), where launch-level parallelism
 comes from the other matrices and per-item overheads dominate.  Labels come from exhaustive timing
-sweeps of this library (`sweep`), which also judge the tuner's regret
+sweeps of this library (`sweep`), which also measure the tuner's regret
 (block_statistics/consistency.cpp:147-149: (t_pick - t_min) / t_min * 100).
 
 Features (the statistics block_statistics/ computes, block_stats.cpp:78-100,
@@ -247,7 +247,7 @@ def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
     return np.array(rows), per_s
 
 
-# ---- exhaustive timing sweep (labels and regret judge) -------------------------
+# ---- exhaustive timing sweep (labels and regret) -------------------------------
 
 def build_candidate(A, s, variant, d_rp, d_c, d_v, stream):
     """The matrix object of one candidate (ABHSF at (s, variant), or the
