@@ -154,6 +154,38 @@ def test_spmm(gpu, s, nrhs):
     assert np.array_equal(got[:, nrhs:], Y0[:, nrhs:])  # padding columns untouched
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("s", [16, 64, 256])
+@pytest.mark.parametrize("nrhs", [1, 3, 8, 13])
+def test_spmm_csr_blocks(gpu, precision, s, nrhs):
+    """27-point stencil on 24^3: CSR blocks (s = 64, 256) whose rows each lane
+    walks two at a time, plus bitmap blocks (s = 16); 13 right-hand sides
+    make a column tile of 8 and one of 5.  Integer values and X keep every
+    sum exact, so Y must equal the oracle bit for bit; then alpha/beta."""
+    A = synth.stencil27(24)
+    print(f"case spmm csr s={s} nrhs={nrhs} {precision}", flush=True)
+    rng = np.random.default_rng(s + nrhs)
+    vi = (rng.integers(1, 9, A.nnz) * rng.choice([-1.0, 1.0], A.nnz)).astype(np.float64)
+    M = AbhsfMatrix(block_size=s, precision=precision)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, vi)
+    X = rng.integers(-8, 9, (A.n, nrhs)).astype(np.float64)
+    dX, dY = DeviceArray.from_numpy(X), DeviceArray.zeros((A.m, nrhs))
+    M.spmm_async(nrhs, 1.0, dX, nrhs, 0.0, dY, nrhs)
+    M.synchronize()
+    got = dY.numpy()
+    for j in range(nrhs):
+        assert np.array_equal(got[:, j], pyoracle.csr_spmv(A.rowptr, A.col, vi, X[:, j].copy())), j
+    Xf = rng.uniform(-1, 1, (A.n, nrhs))
+    Y0 = rng.uniform(-1, 1, (A.m, nrhs))
+    dX, dY = DeviceArray.from_numpy(Xf), DeviceArray.from_numpy(Y0)
+    M.spmm_async(nrhs, -0.5, dX, nrhs, 2.0, dY, nrhs)
+    M.synchronize()
+    got = dY.numpy()
+    for j in range(nrhs):
+        ref = pyoracle.csr_spmv(A.rowptr, A.col, vi, Xf[:, j].copy(), -0.5, 2.0, Y0[:, j].copy())
+        assert relerr(got[:, j], ref) <= TOL, j
+
+
 @pytest.mark.parametrize("s", [8, 32, 64])
 def test_msmf_policy(gpu, s):
     A = synth.block_mix(4096, seed=11)
