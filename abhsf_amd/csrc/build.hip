@@ -397,6 +397,7 @@ SpmvLaunch Storage::launch() const
     L.bal.n_heavy = n_heavy;
     L.bal.n_slots = n_slots;
     L.bal.tiled = tiled;
+    L.bal.tile_zs1 = tile_zs1;
     L.bal.coo_rng = bal_coo_rng.get<uint32_t>();
     return L;
 }
@@ -458,6 +459,7 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels, int tile_r
     bal_order.release();
     bal_coo_rng.release();
     n_light = n_units = n_heavy = n_slots = 0;
+    tile_zs1 = false;
     if (unit <= 0 || M == 0) return;
     ABHSF_REQUIRE(panels == 1 || panels == kXcdCount, ABHSF_ERR_INVALID, "column panels must match the XCD count");
     ABHSF_REQUIRE(tile_rows <= 0 || panels == 1, ABHSF_ERR_INVALID, "tiles and column panels do not combine");
@@ -641,6 +643,21 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels, int tile_r
                 open_work = cw;
             }
         }
+        // blocks of 3 or more entries among the tiles' COO blocks (z - 1 >= 2):
+        // at one in 64 or more, most 64-block sub-chunks run phase B anyway
+        // and phase A takes one entry per block (R-MAT scale 22, s = 256:
+        // 9 % of them, 0.527 -> 0.51 ms; uniform scatter: none, and one entry
+        // per block would send the 0.15 % of blocks with two to phase B)
+        uint64_t tile_blocks = 0, tile_ge3 = 0;
+        for (const auto &it : items) {
+            if (it.size() < 2) continue;
+            for (uint32_t I : it)
+                for (uint64_t b = sp[COO][I]; b < sp[COO][I + 1]; b++) {
+                    tile_blocks++;
+                    tile_ge3 += zz[COO][b] >= 2u;
+                }
+        }
+        tile_zs1 = tile_blocks > 0 && 64 * tile_ge3 >= tile_blocks;
         // tiles first: they start together and sweep x in step (their COO
         // streams run in block-column order), the single strips fill the
         // remaining slots and follow

@@ -1372,6 +1372,13 @@ void launch_strip(const DevMat &A, const Balance &b, int64_t nblk, bool coo_sing
             return;
         }
     }
+    if constexpr (TILED) {
+        if (b.tile_zs1) {
+            hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 4, true>), dim3((unsigned)nblk), dim3(256), 0,
+                               st, A, x, y, alpha, beta, b);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 2, 4, TILED>), dim3((unsigned)nblk), dim3(256), 0, st, A,
                        x, y, alpha, beta, b);
 }

@@ -28,6 +28,10 @@ struct Balance {
     // in (block column, strip) order.
     bool tiled = false;
     const uint32_t *coo_rng = nullptr;  // [n_light][3]
+    // the tiles' COO blocks hold 3 or more entries often enough (one in 64
+    // blocks or more) that phase B runs in most 64-block sub-chunks anyway:
+    // phase A then takes one entry per block instead of two
+    bool tile_zs1 = false;
 };
 
 #ifndef ABHSF_TILE_WORK

@@ -26,6 +26,7 @@ struct Storage {
     // blocks are stored in (block column, strip) order; COO z1 holds z - 1 in
     // its low coo_zb bits and the strip within the tile above them
     bool tiled = false;
+    bool tile_zs1 = false;  // tiles decode one COO entry per block in phase A (see Balance)
     int coo_zb = 16;
     DevBuf bal_coo_rng;  // tiled: [n_light][3] COO {first block, end block, first value} of each item
     // footprints (bits) under both cost models
