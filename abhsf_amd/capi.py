@@ -17,6 +17,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_UNSUPPORTED = range(6)
 POLICY_PREPROC, POLICY_MSMF = 0, 1
 F64, F32 = 0, 1
 VARIANT_AUTO, VARIANT_STRIP, VARIANT_BALANCED, VARIANT_PANEL, VARIANT_TILED = 0, 1, 2, 3, 4
+VARIANT_NAMES = {0: "auto", 1: "strip", 2: "balanced", 3: "panel", 4: "tiled"}
 GENERAL, SYMMETRIC, SKEW_SYMMETRIC = 0, 1, 2
 SCHEMES = ("coo", "csr", "bitmap", "dense")
 
@@ -49,6 +50,7 @@ class Stats(C.Structure):
         ("smf_bits_preproc", C.c_uint64), ("smf_bits_msmf", C.c_uint64),
         ("stored_bytes", C.c_uint64), ("index_bytes", C.c_uint64),
         ("build_ms", C.c_double),
+        ("variant", C.c_int32), ("tile_zs1", C.c_int32), ("work_items", C.c_int64),
     ]
 
     def as_dict(self):
@@ -60,6 +62,8 @@ class Stats(C.Structure):
             "smf_bits_preproc": int(self.smf_bits_preproc), "smf_bits_msmf": int(self.smf_bits_msmf),
             "stored_bytes": int(self.stored_bytes), "index_bytes": int(self.index_bytes),
             "build_ms": float(self.build_ms),
+            "variant": VARIANT_NAMES.get(int(self.variant), int(self.variant)),
+            "tile_zs1": bool(self.tile_zs1), "work_items": int(self.work_items),
         }
 
 

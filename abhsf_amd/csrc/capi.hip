@@ -281,6 +281,9 @@ abhsf_status abhsf_get_stats(const abhsf_handle *h, abhsf_stats *out)
         out->stored_bytes = S.stored_bytes();
         out->index_bytes = S.stored_bytes() - S.value_bytes();
         out->build_ms = S.build_ms;
+        out->variant = h->opts.variant == ABHSF_VARIANT_AUTO ? ABHSF_VARIANT_BALANCED : h->opts.variant;
+        out->tile_zs1 = S.tile_zs1 ? 1 : 0;
+        out->work_items = h->opts.variant == ABHSF_VARIANT_STRIP ? S.M : S.n_light + S.n_units;
     });
 }
 

@@ -36,6 +36,11 @@ namespace {
 template <typename T>
 __device__ inline T ld(const void *p, uint64_t i) { return static_cast<const T *>(p)[i]; }
 
+// x may be any double pointer (x + 1 of a caller's array included); the
+// paired loads of the dense paths take it as double2 only when it is 16-byte
+// aligned.
+__device__ inline bool x_aligned16(const double *x) { return (reinterpret_cast<uintptr_t>(x) & 15u) == 0; }
+
 struct Acc {
     double *hi;  // [s * R]
     double *lo;  // [s * R] (COMP only)
@@ -338,7 +343,9 @@ __device__ inline void bmdn_r1(const DevMat &A, int sc, uint32_t b0, uint32_t b1
             if (sc == DENSE) {
                 if (valid) {
                     const VT *vp = val + db + pbase;
-                    if (gc0 + (int64_t)cw <= n && (cw & 1) == 0) {
+                    // double2 loads of x need a 16-byte aligned x (a caller may pass
+                    // x + 1 of an array): otherwise the scalar walk below
+                    if (gc0 + (int64_t)cw <= n && (cw & 1) == 0 && x_aligned16(x)) {
 #pragma unroll 4
                         for (uint32_t c = 0; c < cw; c += 2) {
                             const double2 xv = *reinterpret_cast<const double2 *>(x + gc0 + c);
@@ -435,6 +442,7 @@ __device__ inline void dense_coal_r1(const DevMat &A, uint32_t b0, uint32_t b1, 
     const double *__restrict__ val = static_cast<const double *>(A.val[DENSE]);
     const int64_t n = A.n;
     const int cl = 2 * (lane % L);
+    const bool xal = x_aligned16(x);  // wave-uniform: x pairs as double2 loads or two scalar loads
     double a[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) a[q] = 0.0;
@@ -442,7 +450,10 @@ __device__ inline void dense_coal_r1(const DevMat &A, uint32_t b0, uint32_t b1, 
     for (uint32_t b = b0; b < b1; b++) {
         const int64_t c0 = (int64_t)ld<BT>(A.bcol[DENSE], b) * S + cl;
         double x0 = 0.0, x1 = 0.0;  // columns past n hold stored zeros; x is not read there
-        if (c0 + 1 < n) {
+        if (c0 + 1 < n && !xal) {
+            x0 = x[c0];
+            x1 = x[c0 + 1];
+        } else if (c0 + 1 < n) {
             const double2 xv = *reinterpret_cast<const double2 *>(x + c0);
             x0 = xv.x;
             x1 = xv.y;

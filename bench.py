@@ -1,25 +1,36 @@
 """ABHSF SpMV benchmark on one MI355X.
 
-Default workload (the largest single-GPU configuration the metric is quoted
-on): block-structured synthetic matrix, n = 2^22, 32-row strips drawn 20%
-three dense 32x32 blocks / 30% block-tridiagonal of 8x8 sub-blocks / 50% 12
-uniform columns per row, values N(0,1) FP64 (seed 0), x ~ U[-1,1] (seed 0);
-ABHSF at s in {16, 32, 64, 128}, balanced and tiled variants.  A step is one SpMV y += A*x (the reference
-harness's accumulating call, spmv/unified/spmv.cpp:1222-1231) with all inputs
-resident in HBM.
+Default workload (the configuration the target is stated on): R-MAT scale 22
+(a, b, c, d = .57, .19, .19, .05), edge factor 16, duplicates merged, values
+U[-1,1] FP64 (seed 0), x ~ U[-1,1] (seed 0).  Every (s, variant) candidate of
+the learned tuner is built and timed (s in 8..256 x strip / balanced / panel /
+tiled, plus the delta-coded format); value = GFLOP/s (2*nnz per step) at the
+tuner's pick (abhsf_amd/tuner.py, MLP on GPU block statistics), against plain
+ABHSF at fixed s = 64 (balanced) and the exhaustive best, with the regret of
+the pick (block_statistics/consistency.cpp:147-149).  A step is one SpMV
+y += A*x (the reference harness's accumulating call,
+spmv/unified/spmv.cpp:1222-1231) with all inputs resident in HBM.
 
-value = GFLOP/s (2*nnz per step) at the (s, variant) the learned tuner picks
-(abhsf_amd/tuner.py, MLP on GPU block statistics); every measured s is
-reported, and plain ABHSF at fixed s = 64.  The roofline entry is for the SpMV
-kernel at the chosen s: algorithmic bytes per launch = all ABHSF arrays as
-stored + 8n (x) + 16m (y read and written), divided by the kernel's average
-duration from HIP events on the handle's stream.
+The roofline entry is for the SpMV kernels at the pick: algorithmic bytes per
+launch = all ABHSF arrays as stored + 8n (x) + 16m (y read and written),
+divided by the average duration of the launch from HIP events on the
+handle's stream.
 
-Other workloads (--workload): rmat (R-MAT scale 22, edge factor 16; tuned vs
-plain with exhaustive regret), spmm (27-point stencil 160^3, k in
-{1,4,8,16,32}, FP64 and FP32 storage), batch (2000 matrices in one launch).
+The block-structured matrix (n = 2^22, 32-row strips 20% three dense 32x32
+blocks / 30% block-tridiagonal of 8x8 sub-blocks / 50% 12 uniform columns per
+row, N(0,1) values) is measured in the same run at s in {16, 32, 64, 128}
+(key "block_mix"), with its conversion compared to the CPU builder's arrays.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload block_mix]
+cpu_baseline: the oracle's OpenMP ABHSF SpMV on the host cores, whole R-MAT
+matrix at s = 64, 5 warm-up + 20 timed calls, median of 5 repeats,
+OMP_PROC_BIND=close (the protocol of spmv/unified/spmv.cpp:54-55); "c1" is
+the same on the 256^2 5-point Laplacian.
+
+Other workloads (--workload): block_mix alone, spmm (27-point stencil 160^3,
+k in {1,4,8,16,32}, FP64 and FP32 storage), batch (2000 matrices in one
+launch).
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload rmat]
 """
 import argparse
 import ctypes as C
@@ -28,6 +39,10 @@ import math
 import os
 import sys
 import time
+
+# the CPU baseline's thread placement; OpenMP reads it when a library that
+# uses it is first loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 import numpy as np
 
@@ -74,29 +89,48 @@ def time_spmv(M, hip, dx, dy, steps, warmup, stream):
     return e0.elapsed_ms(e1)
 
 
-def cpu_baseline(A, s, x, seconds_target=10.0):
-    """The CPU oracle's ABHSF SpMV (OpenMP over strips) on a bounded sample:
-    the first rows of the same matrix, same s."""
+def cpu_spmv_protocol(A, s, x, warmup=5, iters=20, repeats=5):
+    """The CPU oracle's ABHSF SpMV (OpenMP over strips) on the whole matrix:
+    `warmup` untimed calls, then `repeats` runs of `iters` timed calls (the
+    protocol of spmv/unified/spmv.cpp:54-55); GFLOP/s of the median run."""
     from oracle import pyoracle
-    rows = min(A.m, 1 << 20)
-    rp = A.rowptr[: rows + 1].copy()
-    nnz = int(rp[-1])
-    B = pyoracle.Abhsf(rows, A.n, rp, A.col[:nnz], A.val[:nnz], s.bit_length() - 1)
-    y = np.zeros(rows)
-    B.spmm(x, 1.0, 1.0, y)  # warm-up
     t0 = time.perf_counter()
-    it = 0
-    while True:
+    B = pyoracle.Abhsf(A.m, A.n, A.rowptr, A.col, A.val, s.bit_length() - 1)
+    build_s = time.perf_counter() - t0
+    y = np.zeros(A.m)
+    for _ in range(warmup):
         y = B.spmm(x, 1.0, 1.0, y)
-        it += 1
-        el = time.perf_counter() - t0
-        if el >= seconds_target or it >= 200:
-            break
-    return {
-        "value": round(2.0 * nnz * it / el / 1e9, 3), "unit": "GFLOP/s",
-        "cores": int(pyoracle.lib().orc_num_threads()), "kind": "port",
-        "sample": f"first {rows} rows ({nnz} nnz) of the same matrix, s={s}, {it} SpMV calls in {el:.1f} s",
-    }
+    runs = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            y = B.spmm(x, 1.0, 1.0, y)
+        runs.append(time.perf_counter() - t0)
+    med = float(np.median(runs))
+    return {"value": round(2.0 * A.nnz * iters / med / 1e9, 3), "unit": "GFLOP/s",
+            "ms_per_spmv": round(med / iters * 1e3, 3), "cores": int(pyoracle.lib().orc_num_threads()),
+            "kind": "port", "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "s": s,
+            "protocol": f"{warmup} warm-up + {iters} timed calls, median of {repeats} repeats",
+            "oracle_build_s": round(build_s, 1)}
+
+
+def cpu_baseline(A, s, x, name):
+    """cpu_baseline leg: the whole matrix (no sampling) at block size s, and
+    the 256^2 Laplacian at s = 64 (c1)."""
+    from abhsf_amd import synth
+    out = cpu_spmv_protocol(A, s, x)
+    out["sample"] = f"whole matrix: {name}, {A.nnz} nnz, s={s}"
+    L = synth.laplacian2d(256)
+    xl = np.random.default_rng(0).uniform(-1.0, 1.0, L.n)
+    c1 = cpu_spmv_protocol(L, 64, xl)
+    ref = csr_spmv_numpy(L, xl)
+    from oracle import pyoracle
+    B = pyoracle.Abhsf(L.m, L.n, L.rowptr, L.col, L.val, 6)
+    got = B.spmm(xl, 1.0, 0.0, np.zeros(L.m))
+    c1["rel_err_vs_csr"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    c1["sample"] = f"256^2 5-point Laplacian, {L.nnz} nnz, s=64, x ~ U[-1,1] seed 0"
+    out["c1"] = c1
+    return out
 
 
 def csr_spmv_numpy(A, x):
@@ -163,17 +197,38 @@ def tuned_pick(A, allowed_s, hip):
     return cands[best], tuner.NAME
 
 
+def kernel_source_hash():
+    """SHA-256 (first 16 hex digits) of the kernel and build sources, in name
+    order: a stored counter profile applies to this build only if it was
+    taken at the same hash."""
+    import hashlib
+    d = os.path.join(ROOT, "abhsf_amd", "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".hpp", ".cpp", ".c")) or f == "Makefile":
+            h.update(f.encode())
+            h.update(open(os.path.join(d, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(key):
     """HBM bytes per launch of the SpMV kernel for this configuration, from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
     profiles/pmc_traffic.json (each counter scaled by what it reads for the
-    plain copy kernel of known size); None when that configuration was not
-    profiled."""
+    plain copy kernel of known size).  Returns (bytes, note): bytes is None
+    when the configuration was not profiled or was profiled at other kernel
+    sources (the note says which)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, "no counter profile"
     rec = json.load(open(path)).get(key)
-    return None if rec is None else int(rec["traffic_bytes"])
+    if rec is None:
+        return None, f"{key} not profiled"
+    src = kernel_source_hash()
+    if rec.get("source_sha") != src:
+        return None, f"profile of {key} taken at sources {rec.get('source_sha')}, these are {src}"
+    return int(rec["traffic_bytes"]), (f"rocprofv3 FETCH_SIZE x {rec['fetch_factor']} + WRITE_SIZE x "
+                                       f"{rec['write_factor']} at sources {src} (profiles/pmc_traffic.json)")
 
 
 def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
@@ -208,7 +263,8 @@ def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
     write = sum(w[k]["WRITE_SIZE"] for k in ("spmv", "fixup") if k in w) * 1024
     rec = {"fetch_kib": round(fetch / 1024, 1), "write_kib": round(write / 1024, 1),
            "fetch_factor": round(f_factor, 4), "write_factor": round(w_factor, 4),
-           "traffic_bytes": int(fetch * f_factor + write * w_factor), "command": command}
+           "traffic_bytes": int(fetch * f_factor + write * w_factor), "command": command,
+           "source_sha": kernel_source_hash()}
     if sizes_csv:
         z = per_kernel(sizes_csv)
 
@@ -230,8 +286,9 @@ def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
 
 
 def roofline(r, key=None):
+    traffic, note = measured_traffic(key) if key else (None, "not profiled")
     return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(key) if key else None,
+            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": note,
             "algorithmic_bytes": r.get("algo_bytes")}
 
 
@@ -278,7 +335,7 @@ def run_block_mix(args, hip):
         out["copy_calibration"] = calib_copy_gbs(hip)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(A, pick[0], x)
+        out["cpu_baseline"] = cpu_baseline(A, 64, x, "block_mix n=2^22")
     return out
 
 
@@ -309,9 +366,20 @@ def run_rmat(args, hip):
     out["plain_s64"] = {"gflops": plain["gflops"], "gbs": plain["gbs"]}
     out["tuned_vs_plain"] = round(best["gflops"] / plain["gflops"], 3)
     out["regret_pct"] = round(tuner.regret(times, pick), 2) if len(times) > 1 else None
+    top = max(res, key=lambda k: res[k]["gflops"])
+    out["exhaustive_best"] = {"s": top[0], "variant": top[1], "gflops": res[top]["gflops"], "ms": res[top]["ms"]}
     out["per_candidate"] = {f"{s}/{v}": r for (s, v), r in res.items()}
     if not args.no_calib:
         out["copy_calibration"] = calib_copy_gbs(hip)
+    if not args.no_extra and args.variant is None:
+        log("block_mix n=2^22 ...")
+        bm = run_block_mix(argparse.Namespace(**{**vars(args), "no_cpu_baseline": True, "no_calib": True,
+                                                 "s": None, "variant": None}), hip)
+        out["block_mix"] = {k: bm[k] for k in ("value", "ms_per_step", "config", "roofline", "plain_s64",
+                                               "tuned_vs_plain", "exhaustive_best", "per_s")}
+    if not args.no_cpu_baseline and args.variant is None:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(A, 64, x, "R-MAT scale 22 ef 16")
     return out
 
 
@@ -443,7 +511,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="block_mix", choices=["block_mix", "rmat", "spmm", "batch"])
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "block_mix", "spmm", "batch"])
     ap.add_argument("--s", default=None, help="block sizes to measure (block_mix default 16,32,64,128; "
                     "rmat default all tuner candidates)")
     ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel", "tiled", "delta"],
@@ -454,6 +522,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-calib", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="rmat: skip the block_mix measurement")
     ap.add_argument("--pmc-summary", nargs="+", metavar="ARG",
                     help="KEY FETCH_CSV WRITE_CSV [SIZES_CSV]: summarise rocprofv3 --pmc counter CSVs of a "
                          "profiled bench run into profiles/pmc_traffic.json and exit (no GPU needed)")

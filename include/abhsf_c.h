@@ -109,6 +109,10 @@ typedef struct abhsf_stats {
     uint64_t stored_bytes;         /* bytes of all storage arrays as stored in HBM */
     uint64_t index_bytes;          /* stored_bytes minus value bytes */
     double build_ms;               /* device time of the last build */
+    int32_t variant;               /* ABHSF_VARIANT_* the storage was built for (AUTO resolved) */
+    int32_t tile_zs1;              /* tiled: 1 if tile COO decode takes one entry per block in its
+                                      first phase (blocks of 3+ entries are 1 in 64 or more), else 0 */
+    int64_t work_items;            /* single-RHS work items: light items + heavy-strip units */
 } abhsf_stats;
 
 typedef struct abhsf_handle abhsf_handle;

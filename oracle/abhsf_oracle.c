@@ -402,6 +402,10 @@ int orc_build(orc_abhsf *A, int64_t m, int64_t n, const int64_t *rowptr, const i
     uint64_t smf = 0;
     uint64_t hdr = orc_ceil_log2((uint64_t)A->M) + orc_ceil_log2((uint64_t)A->N) + 2;
     uint32_t *rowcnt = (uint32_t *)malloc((size_t)(s + 1) * 4);
+    /* row of every element: rowof[e] = r for rowptr[r] <= e < rowptr[r + 1] */
+    uint32_t *rowof = (uint32_t *)xcalloc(nnz > 0 ? nnz : 1, 4);
+    for (int64_t r = 0; r < m; r++)
+        for (int64_t e = rowptr[r]; e < rowptr[r + 1]; e++) rowof[e] = (uint32_t)r;
     for (uint64_t b = 0; b < nblk; b++) {
         int x = bS[b];
         uint64_t z = bZ[b], o = ob[x]++, e = oe[x];
@@ -413,13 +417,7 @@ int orc_build(orc_abhsf *A, int64_t m, int64_t n, const int64_t *rowptr, const i
         if (x == ORC_CSR) memset(rowcnt, 0, (size_t)(s + 1) * 4);
         for (uint64_t q = 0; q < z; q++) {
             uint64_t el = perm[bStart[b] + q];
-            /* row of element el: find via rowptr (strip-local binary search) */
-            int64_t lo = 0, hi = m; /* rowptr[lo] <= el < rowptr[hi] */
-            while (hi - lo > 1) {
-                int64_t mid = (lo + hi) / 2;
-                if ((uint64_t)rowptr[mid] <= el) lo = mid; else hi = mid;
-            }
-            uint64_t lrow = (uint64_t)lo & (uint64_t)(s - 1), lcol = (uint64_t)col[el] & (uint64_t)(s - 1);
+            uint64_t lrow = (uint64_t)rowof[el] & (uint64_t)(s - 1), lcol = (uint64_t)col[el] & (uint64_t)(s - 1);
             double v = val ? val[el] : 1.0;
             switch (x) {
             case ORC_COO:
@@ -454,6 +452,7 @@ int orc_build(orc_abhsf *A, int64_t m, int64_t n, const int64_t *rowptr, const i
         smf += (uint64_t)A->M * orc_ceil_log2((uint64_t)A->N + 1) + nblk * orc_ceil_log2((uint64_t)A->N);
     A->smf_bits = smf;
     free(rowcnt);
+    free(rowof);
     free(perm); free(bJ); free(bZ); free(bStart); free(bS); free(stripBlk);
     return 0;
 }

@@ -7,6 +7,7 @@ FP64 oracle on the FP32-rounded values <= 1e-12 (compensated accumulation),
 and the error against the FP64 values is reported.
 """
 import functools
+import os
 
 import numpy as np
 import pytest
@@ -284,8 +285,27 @@ def test_device_csr_input_and_determinism(gpu):
         outs.append(dy.numpy())
     ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)
     assert relerr(outs[0], ref) <= TOL
+    # FP64 results are deterministic: each strip accumulates in one
+    # wavefront's LDS, and heavy strips' partials are summed in a fixed order
     same = all(np.array_equal(outs[0], o) for o in outs[1:])
-    print(f"bitwise-identical repeated SpMV: {same}")
+    assert same, "repeated FP64 SpMV differs bit for bit"
+
+
+@pytest.mark.parametrize("variant", ["balanced", "tiled"])
+def test_repeated_spmv_bitwise_identical(gpu, variant):
+    """Heavy strips (units + fix-up) and tiles: three runs, same bits."""
+    A = synth.rmat(15, 16, seed=4)
+    print(f"case determinism rmat15 s=64 {variant}", flush=True)
+    M = AbhsfMatrix(block_size=64, variant=variant)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    dx = DeviceArray.from_numpy(np.random.default_rng(1).uniform(-1, 1, A.n))
+    outs = []
+    for _ in range(3):
+        dy = DeviceArray.zeros(A.m)
+        M.spmv_async(1.0, dx, 0.0, dy)
+        M.synchronize()
+        outs.append(dy.numpy())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
 
 @pytest.mark.parametrize("variant", ["strip", "balanced", "panel"])
@@ -385,3 +405,57 @@ def test_panel_variant(gpu, name, s):
     for j in range(3):
         ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, X[:, j].copy())
         assert relerr(Y[:, j], ref) <= TOL, j
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("s", [16, 32, 64, 128])
+def test_unaligned_x(gpu, precision, s):
+    """x may start at any double (here x + 1 of a device array, 8 bytes past a
+    16-byte boundary): dense-scheme blocks, whose paths load x in pairs, give
+    the same result as from an aligned copy."""
+    A = full_matrix(300, 301)  # dense-scheme blocks everywhere
+    print(f"case unaligned x s={s} {precision}", flush=True)
+    M = AbhsfMatrix(block_size=s, precision=precision)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    assert M.stats()["blocks"]["dense"] > 0
+    x = np.random.default_rng(2).uniform(-1, 1, A.n)
+    buf = DeviceArray.from_numpy(np.concatenate([[0.0], x]))  # x at byte offset 8
+    assert (buf.address + 8) % 16 == 8
+    dx = DeviceArray.from_numpy(x)
+    outs = []
+    for xaddr in (dx.address, buf.address + 8):
+        dy = DeviceArray.zeros(A.m)
+        M.spmv_async(1.0, xaddr, 0.0, dy)
+        M.synchronize()
+        outs.append(dy.numpy())
+    assert np.array_equal(outs[0], outs[1])
+    vals = A.val if precision == "f64" else A.val.astype(np.float32).astype(np.float64)
+    assert relerr(outs[1], pyoracle.csr_spmv(A.rowptr, A.col, vals, x)) <= TOL
+
+
+@pytest.mark.parametrize("case", ["test.mtx", "block_mix_2^13", "rmat_12", "laplacian_96"])
+def test_smf_bits_msmf_pinned(gpu, case):
+    """abhsf_stats.smf_bits_msmf is the structure footprint of
+    block_statistics/mmf.cpp (top-level block CSR M ceil(log2(N+1)) +
+    NNZB ceil(log2 N) plus, per nonzero block, the cheapest scheme + 2 bits):
+    the oracle's mmf row (pinned to the reference's mmf binary) minus the
+    value bits, at every s.  test.mtx at s = 256 is App. B's known answer
+    (CSR-ABHSF 259 bits)."""
+    if case == "test.mtx":
+        m, n, r, c, v = read_mtx(os.path.join(ROOT, "tests", "golden", "test.mtx"))
+        A = synth.from_triplets(m, n, r, c, v)
+    else:
+        A = {"block_mix_2^13": lambda: synth.block_mix(1 << 13, 7), "rmat_12": lambda: synth.rmat(12, 8, seed=2),
+             "laplacian_96": lambda: synth.laplacian2d(96)}[case]()
+    rows, cols, _ = A.triplets()
+    print(f"case smf_bits_msmf {case}", flush=True)
+    for k in range(1, 9):
+        s = 1 << k
+        M = AbhsfMatrix(block_size=s)
+        M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+        got = M.stats()["smf_bits_msmf"]
+        M.release()
+        row = pyoracle.mmf_row(A.m, A.n, A.nnz, s, s, pyoracle.block_hist(rows, cols, k, k))
+        assert got == row[6] - A.nnz * 64, (case, s)
+        if case == "test.mtx" and s == 256:
+            assert got == 259

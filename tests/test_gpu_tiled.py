@@ -105,6 +105,32 @@ def test_tiled_heavy_strips(gpu, precision, s):
     assert relerr(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("case", ["scatter_zs2", "dense_coo_zs1"])
+def test_tiled_phase_a_shape(gpu, precision, case):
+    """The tiles' COO decode takes two entries per block in its first phase,
+    or one when blocks of 3+ entries are at least 1 in 64 of the tiles'
+    blocks (stats()["tile_zs1"]).  One matrix on each side of that threshold,
+    in both value precisions, checked exactly on integer data."""
+    if case == "scatter_zs2":  # 12 uniform columns per row: tile blocks hold 1 or 2 entries
+        A, s, want = synth.block_mix(1 << 18, 2, 0.0, 0.0), 32, False
+    else:  # ~10 entries per 32 x 32 block, all COO, strips of ~1300 entries
+        A, s, want = synth.random_sparse(4096, 4096, 0.01, seed=12), 32, True
+    print(f"case tiled phase-A {case} {precision}", flush=True)
+    Ai = integer_copy(A, 3)
+    M = AbhsfMatrix(block_size=s, precision=precision, variant="tiled")
+    M.from_csr(Ai.m, Ai.n, Ai.rowptr, Ai.col, Ai.val)
+    st = M.stats()
+    assert st["variant"] == "tiled"
+    assert st["tile_zs1"] is want
+    xi = np.random.default_rng(4).integers(-8, 9, A.n).astype(np.float64)
+    got = run(M, xi, np.zeros(A.m), 1.0, 0.0)
+    ref = pyoracle.csr_spmv(Ai.rowptr, Ai.col, Ai.val, xi)
+    # integers up to 8 * 8 * nnz-per-row: exact in FP64 and in the f32-valued
+    # storage (the values are small integers, exact in FP32)
+    assert np.array_equal(got, ref), "tiled SpMV is not exact on integer data"
+
+
 def test_tiled_in_batch(gpu):
     st = hip.Stream()
     mats = [(synth.block_mix(1 << 13, 5, 0.0, 0.0), 32, "tiled"), (synth.rmat(13, 8, seed=3), 64, "tiled"),
