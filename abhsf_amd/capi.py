@@ -25,7 +25,7 @@ SCHEMES = ("coo", "csr", "bitmap", "dense")
 EXPORTED = (
     "abhsf_create", "abhsf_build_csr", "abhsf_build_coo", "abhsf_get_stats", "abhsf_spmv",
     "abhsf_spmm", "abhsf_export", "abhsf_import", "abhsf_block_hist", "abhsf_mlp_num_params", "abhsf_mlp_forward",
-    "abhsf_mlp_backward", "abhsf_mlp_train", "abhsf_batch_create", "abhsf_batch_set_vectors",
+    "abhsf_mlp_backward", "abhsf_mlp_train", "abhsf_mlp_train_noalloc", "abhsf_batch_create", "abhsf_batch_set_vectors",
     "abhsf_batch_spmv", "abhsf_batch_synchronize", "abhsf_batch_profile", "abhsf_batch_destroy", "abhsf_synchronize",
     "abhsf_destroy", "abhsf_last_error", "abhsf_version", "abhsf_set_symmetry",
     "abhsf_delta_create", "abhsf_delta_build_csr", "abhsf_delta_spmv", "abhsf_delta_get_stats",
@@ -113,6 +113,7 @@ def lib():
         L.abhsf_mlp_forward.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp]
         L.abhsf_mlp_backward.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp, vp, vp]
         L.abhsf_mlp_train.argtypes = [vp, i32, vp, vp, vp, i64, i64, C.c_float, vp, vp]
+        L.abhsf_mlp_train_noalloc.argtypes = [vp, i32, vp, vp, vp, i64, i64, C.c_float, vp, vp]
         L.abhsf_batch_create.argtypes = [vp, i32, vp, C.POINTER(vp)]
         L.abhsf_batch_set_vectors.argtypes = [vp, vp, vp]
         L.abhsf_batch_spmv.argtypes = [vp, d, d, i32]

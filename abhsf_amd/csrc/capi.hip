@@ -24,6 +24,9 @@ extern "C" abhsf_status abhsf_mlp_backward_impl(const float *theta, int32_t F, c
 extern "C" abhsf_status abhsf_mlp_train_impl(float *theta, int32_t F, const float *X, const float *y, const float *w,
                                              int64_t B, int64_t steps, float lr, float *loss_host, void *stream,
                                              std::string *err);
+extern "C" abhsf_status abhsf_mlp_train_noalloc_impl(float *theta, int32_t F, const float *X, const float *y,
+                                                     const float *w, int64_t B, int64_t steps, float lr,
+                                                     float *loss_dev, void *stream, std::string *err);
 
 struct abhsf_handle {
     abhsf_options opts;
@@ -395,6 +398,17 @@ abhsf_status abhsf_mlp_train(float *theta, int32_t nfeat, const float *X, const 
 {
     std::string err;
     const abhsf_status st = abhsf_mlp_train_impl(theta, nfeat, X, y, w, nsamples, steps, lr, loss_host, stream, &err);
+    if (st != ABHSF_OK) return fail(st, err);
+    g_last_error.clear();
+    return ABHSF_OK;
+}
+
+abhsf_status abhsf_mlp_train_noalloc(float *theta, int32_t nfeat, const float *X, const float *y, const float *w,
+                                     int64_t nsamples, int64_t steps, float lr, float *loss, void *stream)
+{
+    std::string err;
+    const abhsf_status st =
+        abhsf_mlp_train_noalloc_impl(theta, nfeat, X, y, w, nsamples, steps, lr, loss, stream, &err);
     if (st != ABHSF_OK) return fail(st, err);
     g_last_error.clear();
     return ABHSF_OK;

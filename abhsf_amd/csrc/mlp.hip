@@ -13,6 +13,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace abhsf {
@@ -77,12 +78,15 @@ __global__ __launch_bounds__(kT) void k_mlp_fwd(const float *__restrict__ theta,
 }
 
 // Per-sample deltas: D3 = dL/dout, D2 = dL/dz2, D1 = dL/dz1.
+// wsum_p (device, may be null) replaces wsum when set.
 __global__ __launch_bounds__(kT) void k_mlp_delta(const float *__restrict__ theta, int F, int64_t B,
                                                   const float *__restrict__ H1, const float *__restrict__ H2,
                                                   const float *__restrict__ out, const float *__restrict__ y,
-                                                  const float *__restrict__ w, float wsum, float *__restrict__ D1,
+                                                  const float *__restrict__ w, float wsum,
+                                                  const float *__restrict__ wsum_p, float *__restrict__ D1,
                                                   float *__restrict__ D2, float *__restrict__ D3)
 {
+    if (wsum_p) wsum = *wsum_p;
     __shared__ float sW2[H * H + H];
     const Layout L = layout(F);
     for (int i = threadIdx.x; i < H * H + H; i += blockDim.x) sW2[i] = theta[L.W2 + i];  // W2 and b2, then W3
@@ -155,8 +159,9 @@ __global__ __launch_bounds__(kT) void k_adam(float *__restrict__ theta, const fl
 // Weighted squared error sum, one workgroup, fixed order.
 __global__ __launch_bounds__(kT) void k_loss(const float *__restrict__ out, const float *__restrict__ y,
                                              const float *__restrict__ w, int64_t B, float wsum,
-                                             float *__restrict__ loss)
+                                             const float *__restrict__ wsum_p, float *__restrict__ loss)
 {
+    if (wsum_p) wsum = *wsum_p;
     __shared__ float red[kT];
     float a = 0.f;
     for (int64_t b = threadIdx.x; b < B; b += kT) {
@@ -172,12 +177,64 @@ __global__ __launch_bounds__(kT) void k_loss(const float *__restrict__ out, cons
     if (threadIdx.x == 0) *loss = red[0] / wsum;
 }
 
+// Sum of the sample weights in sample order, in double, rounded to float:
+// the same value the host computes for abhsf_mlp_train.  One thread.
+__global__ void k_wsum(const float *__restrict__ w, int64_t B, float *__restrict__ wsum)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int64_t b = 0; b < B; b++) s += (double)w[b];
+    *wsum = (float)s;
+}
+
+__global__ __launch_bounds__(kT) void k_fill0(float *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
 inline unsigned g(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+// Training work buffer of abhsf_mlp_train_noalloc: a file-scope device array
+// (one copy per GPU, held until the process ends) instead of a hipMalloc per
+// call.  Layout per call: H1, H2, D1, D2 [B][64], out, D3 [B], grad, m, v [P],
+// loss [steps], wsum [1].
+constexpr int64_t kNoAllocMaxSamples = 4096;
+constexpr int64_t kNoAllocMaxSteps = 16384;
+constexpr int64_t kMaxParams = (int64_t)H * kMaxF + H + (int64_t)H * H + H + H + 1;
+constexpr int64_t kScratchFloats =
+    kNoAllocMaxSamples * (4 * H + 2) + 3 * kMaxParams + kNoAllocMaxSteps + 1;
+static_assert(kScratchFloats * 4 < (8ll << 20), "scratch stays under 8 MiB per GPU");
 
 }  // namespace
 }  // namespace abhsf
 
 using namespace abhsf;
+
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (each GPU holds its own copy at its own
+// address; the host-side name is not a device address).
+std::mutex g_scratch_mu;
+float *g_scratch_dev[64] = {};
+
+float *scratch_on_current_device()
+{
+    int dev = 0;
+    ABHSF_HIP(hipGetDevice(&dev));
+    ABHSF_REQUIRE(dev >= 0 && dev < 64, ABHSF_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        ABHSF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ABHSF_REQUIRE(p != nullptr, ABHSF_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[dev] = static_cast<float *>(p);
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
 
 extern "C" int64_t abhsf_mlp_num_params(int32_t F) { return (F >= 1 && F <= kMaxF) ? layout(F).P : -1; }
 
@@ -208,7 +265,7 @@ extern "C" abhsf_status abhsf_mlp_backward_impl(const float *theta, int32_t F, c
         ABHSF_REQUIRE(F >= 1 && F <= kMaxF, ABHSF_ERR_UNSUPPORTED, "feature count must be in [1, 64]");
         ABHSF_REQUIRE(wsum > 0.f, ABHSF_ERR_INVALID, "weights must sum to a positive value");
         hipStream_t st = static_cast<hipStream_t>(stream);
-        k_mlp_delta<<<g(B), kT, 0, st>>>(theta, F, B, H1, H2, out, y, w, wsum, D1, D2, D3);
+        k_mlp_delta<<<g(B), kT, 0, st>>>(theta, F, B, H1, H2, out, y, w, wsum, nullptr, D1, D2, D3);
         ABHSF_HIP(hipGetLastError());
         k_mlp_grad<<<g(layout(F).P), kT, 0, st>>>(F, X, B, H1, H2, D1, D2, D3, grad);
         ABHSF_HIP(hipGetLastError());
@@ -247,8 +304,8 @@ extern "C" abhsf_status abhsf_mlp_train_impl(float *theta, int32_t F, const floa
             const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
             for (int64_t t = 1; t <= steps; t++) {
                 k_mlp_fwd<<<g(B), kT, 0, st>>>(theta, F, X, B, H1, H2, out);
-                k_loss<<<1, kT, 0, st>>>(out, y, w, B, wsum, loss + (t - 1));
-                k_mlp_delta<<<g(B), kT, 0, st>>>(theta, F, B, H1, H2, out, y, w, wsum, D1, D2, D3);
+                k_loss<<<1, kT, 0, st>>>(out, y, w, B, wsum, nullptr, loss + (t - 1));
+                k_mlp_delta<<<g(B), kT, 0, st>>>(theta, F, B, H1, H2, out, y, w, wsum, nullptr, D1, D2, D3);
                 k_mlp_grad<<<g(L.P), kT, 0, st>>>(F, X, B, H1, H2, D1, D2, D3, grad);
                 const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
                 k_adam<<<g(L.P), kT, 0, st>>>(theta, grad, m, v, L.P, lr, b1, b2, eps, c1, c2);
@@ -263,6 +320,57 @@ extern "C" abhsf_status abhsf_mlp_train_impl(float *theta, int32_t F, const floa
             throw;
         }
         ABHSF_HIP(hipFree(buf));
+        return ABHSF_OK;
+    } catch (const Error &e) {
+        *err = e.what();
+        return e.code;
+    }
+}
+
+// abhsf_mlp_train without allocation: the same kernels in the same order on
+// the same data, so theta and the losses equal abhsf_mlp_train's bit for bit;
+// the work buffer is the device array `scratch`.  Asynchronous on stream (no
+// host synchronisation, no memset: m and v are cleared by a fill kernel), so
+// it can be captured into a graph.  Calls that overlap on one GPU share the
+// array and race.
+extern "C" abhsf_status abhsf_mlp_train_noalloc_impl(float *theta, int32_t F, const float *X, const float *y,
+                                                     const float *w, int64_t B, int64_t steps, float lr,
+                                                     float *loss_dev, void *stream, std::string *err)
+{
+    try {
+        ABHSF_REQUIRE(theta && X && y && B >= 1 && steps >= 0, ABHSF_ERR_INVALID, "invalid argument");
+        ABHSF_REQUIRE(F >= 1 && F <= kMaxF, ABHSF_ERR_UNSUPPORTED, "feature count must be in [1, 64]");
+        ABHSF_REQUIRE(B <= kNoAllocMaxSamples, ABHSF_ERR_UNSUPPORTED,
+                      "abhsf_mlp_train_noalloc: nsamples " + std::to_string(B) + " exceeds the scratch limit of " +
+                          std::to_string(kNoAllocMaxSamples) + " samples");
+        ABHSF_REQUIRE(steps <= kNoAllocMaxSteps, ABHSF_ERR_UNSUPPORTED,
+                      "abhsf_mlp_train_noalloc: steps " + std::to_string(steps) + " exceeds the scratch limit of " +
+                          std::to_string(kNoAllocMaxSteps) + " steps");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const Layout L = layout(F);
+        float *buf = scratch_on_current_device();
+        float *H1 = buf, *H2 = H1 + B * H, *D1 = H2 + B * H, *D2 = D1 + B * H, *out = D2 + B * H, *D3 = out + B;
+        float *grad = D3 + B, *m = grad + L.P, *v = m + L.P, *loss = v + L.P, *wsum_p = loss + steps;
+        if (loss_dev) loss = loss_dev;
+        const float wsum = (float)B;  // unit weights; with w the device sum replaces it
+        if (w) {
+            k_wsum<<<1, 64, 0, st>>>(w, B, wsum_p);
+            ABHSF_HIP(hipGetLastError());
+        } else {
+            wsum_p = nullptr;
+        }
+        k_fill0<<<g(2 * L.P), kT, 0, st>>>(m, 2 * L.P);
+        ABHSF_HIP(hipGetLastError());
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        for (int64_t t = 1; t <= steps; t++) {
+            k_mlp_fwd<<<g(B), kT, 0, st>>>(theta, F, X, B, H1, H2, out);
+            k_loss<<<1, kT, 0, st>>>(out, y, w, B, wsum, wsum_p, loss + (t - 1));
+            k_mlp_delta<<<g(B), kT, 0, st>>>(theta, F, B, H1, H2, out, y, w, wsum, wsum_p, D1, D2, D3);
+            k_mlp_grad<<<g(L.P), kT, 0, st>>>(F, X, B, H1, H2, D1, D2, D3, grad);
+            const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+            k_adam<<<g(L.P), kT, 0, st>>>(theta, grad, m, v, L.P, lr, b1, b2, eps, c1, c2);
+            ABHSF_HIP(hipGetLastError());
+        }
         return ABHSF_OK;
     } catch (const Error &e) {
         *err = e.what();

@@ -235,6 +235,19 @@ abhsf_status abhsf_mlp_backward(const float *theta, int32_t nfeat, const float *
  * loss before each step.  Synchronous on stream. */
 abhsf_status abhsf_mlp_train(float *theta, int32_t nfeat, const float *X, const float *y, const float *w,
                              int64_t nsamples, int64_t steps, float lr, float *loss_host, void *stream);
+/* abhsf_mlp_train without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving or tuning loops, harnesses that forbid hipMalloc).
+ * The work buffer (activations, deltas, gradient, Adam moments) is a
+ * file-scope device array of at most 4.4 MB per GPU, kept until the process
+ * ends; limits: nsamples <= 4096 and steps <= 16384 (beyond them:
+ * ABHSF_ERR_UNSUPPORTED, nothing enqueued).  Same kernels in the same order
+ * as abhsf_mlp_train, so theta and the losses are bit-identical to its
+ * results.  Differences: asynchronous on stream (no host synchronisation,
+ * no memset); loss is a DEVICE array of `steps` floats, or NULL.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+abhsf_status abhsf_mlp_train_noalloc(float *theta, int32_t nfeat, const float *X, const float *y, const float *w,
+                                     int64_t nsamples, int64_t steps, float lr, float *loss, void *stream);
 
 /* ---- Delta-coded coordinate storage (spmv/unified/spmv.cpp:628-1033) ----
  * The reference's delta_matrix family stores the deltas of linearised keys
