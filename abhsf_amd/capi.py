@@ -19,16 +19,18 @@ F64, F32 = 0, 1
 VARIANT_AUTO, VARIANT_STRIP, VARIANT_BALANCED, VARIANT_PANEL, VARIANT_TILED = 0, 1, 2, 3, 4
 VARIANT_NAMES = {0: "auto", 1: "strip", 2: "balanced", 3: "panel", 4: "tiled"}
 GENERAL, SYMMETRIC, SKEW_SYMMETRIC = 0, 1, 2
+DELTA_VARINT, DELTA_U32_ROW, DELTA_U32_BLOCK = 0, 1, 2
+DELTA_CODECS = {0: "varint", 1: "u32_row", 2: "u32_block"}
 SCHEMES = ("coo", "csr", "bitmap", "dense")
 
 # every symbol include/abhsf_c.h declares
 EXPORTED = (
     "abhsf_create", "abhsf_build_csr", "abhsf_build_coo", "abhsf_get_stats", "abhsf_spmv",
-    "abhsf_spmm", "abhsf_export", "abhsf_import", "abhsf_block_hist", "abhsf_mlp_num_params", "abhsf_mlp_forward",
+    "abhsf_spmm", "abhsf_export", "abhsf_import", "abhsf_block_hist", "abhsf_mlp_num_params", "abhsf_mlp_forward", "abhsf_mlp_predict",
     "abhsf_mlp_backward", "abhsf_mlp_train", "abhsf_mlp_train_noalloc", "abhsf_batch_create", "abhsf_batch_set_vectors",
     "abhsf_batch_spmv", "abhsf_batch_synchronize", "abhsf_batch_profile", "abhsf_batch_destroy", "abhsf_synchronize",
     "abhsf_destroy", "abhsf_last_error", "abhsf_version", "abhsf_set_symmetry",
-    "abhsf_delta_create", "abhsf_delta_build_csr", "abhsf_delta_spmv", "abhsf_delta_get_stats",
+    "abhsf_delta_create", "abhsf_delta_create_codec", "abhsf_delta_build_csr", "abhsf_delta_spmv", "abhsf_delta_get_stats",
     "abhsf_delta_export", "abhsf_delta_synchronize", "abhsf_delta_destroy",
 )
 
@@ -81,12 +83,14 @@ class DeltaStats(C.Structure):
     _fields_ = [
         ("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64), ("key_bits", C.c_int32), ("chunk", C.c_int32),
         ("words", C.c_uint64 * 4), ("stored_bytes", C.c_uint64), ("build_ms", C.c_double),
+        ("codec", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
         return {"m": self.m, "n": self.n, "nnz": self.nnz, "key_bits": self.key_bits, "chunk": self.chunk,
                 "words": {w: int(self.words[i]) for i, w in enumerate((1, 2, 4, 8))},
-                "stored_bytes": int(self.stored_bytes), "build_ms": float(self.build_ms)}
+                "stored_bytes": int(self.stored_bytes), "build_ms": float(self.build_ms),
+                "codec": DELTA_CODECS.get(int(self.codec), int(self.codec))}
 
 
 _lib = None
@@ -111,6 +115,7 @@ def lib():
         L.abhsf_block_hist.argtypes = [vp, vp, i64, i64, i64, i32, i32, i32, i64, vp, i64, vp, vp, vp]
         L.abhsf_mlp_num_params.argtypes = [i32]
         L.abhsf_mlp_forward.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp]
+        L.abhsf_mlp_predict.argtypes = [vp, i32, i32, vp, i64, vp, vp]
         L.abhsf_mlp_backward.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp, vp, vp]
         L.abhsf_mlp_train.argtypes = [vp, i32, vp, vp, vp, i64, i64, C.c_float, vp, vp]
         L.abhsf_mlp_train_noalloc.argtypes = [vp, i32, vp, vp, vp, i64, i64, C.c_float, vp, vp]
@@ -121,6 +126,7 @@ def lib():
         L.abhsf_batch_profile.argtypes = [vp, d, d, vp]
         L.abhsf_set_symmetry.argtypes = [vp, i32]
         L.abhsf_delta_create.argtypes = [vp, C.POINTER(vp)]
+        L.abhsf_delta_create_codec.argtypes = [vp, i32, C.POINTER(vp)]
         L.abhsf_delta_build_csr.argtypes = [vp, i64, i64, i64, vp, vp, vp, i32]
         L.abhsf_delta_spmv.argtypes = [vp, d, vp, d, vp]
         L.abhsf_delta_get_stats.argtypes = [vp, C.POINTER(DeltaStats)]

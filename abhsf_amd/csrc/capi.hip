@@ -24,6 +24,8 @@ extern "C" abhsf_status abhsf_mlp_backward_impl(const float *theta, int32_t F, c
 extern "C" abhsf_status abhsf_mlp_train_impl(float *theta, int32_t F, const float *X, const float *y, const float *w,
                                              int64_t B, int64_t steps, float lr, float *loss_host, void *stream,
                                              std::string *err);
+extern "C" abhsf_status abhsf_mlp_predict_impl(const float *thetas, int32_t nm, int32_t F, const float *X,
+                                               int64_t B, double *out, void *stream, std::string *err);
 extern "C" abhsf_status abhsf_mlp_train_noalloc_impl(float *theta, int32_t F, const float *X, const float *y,
                                                      const float *w, int64_t B, int64_t steps, float lr,
                                                      float *loss_dev, void *stream, std::string *err);
@@ -38,6 +40,7 @@ struct abhsf_handle {
 
 struct abhsf_delta {
     hipStream_t stream = nullptr;
+    int32_t codec = ABHSF_DELTA_VARINT;
     abhsf::DeltaStorage D;
 };
 
@@ -380,6 +383,16 @@ abhsf_status abhsf_mlp_forward(const float *theta, int32_t nfeat, const float *X
     return ABHSF_OK;
 }
 
+abhsf_status abhsf_mlp_predict(const float *thetas, int32_t nmembers, int32_t nfeat, const float *X,
+                               int64_t nsamples, double *out, void *stream)
+{
+    std::string err;
+    const abhsf_status st = abhsf_mlp_predict_impl(thetas, nmembers, nfeat, X, nsamples, out, stream, &err);
+    if (st != ABHSF_OK) return fail(st, err);
+    g_last_error.clear();
+    return ABHSF_OK;
+}
+
 abhsf_status abhsf_mlp_backward(const float *theta, int32_t nfeat, const float *X, int64_t nsamples,
                                 const float *H1, const float *H2, const float *out, const float *y,
                                 const float *w, float wsum, float *D1, float *D2, float *D3, float *grad,
@@ -573,6 +586,18 @@ abhsf_status abhsf_delta_create(void *stream, abhsf_delta **out)
     });
 }
 
+abhsf_status abhsf_delta_create_codec(void *stream, int32_t codec, abhsf_delta **out)
+{
+    return guarded([&] {
+        ABHSF_REQUIRE(out, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(codec == ABHSF_DELTA_VARINT || codec == ABHSF_DELTA_U32_ROW || codec == ABHSF_DELTA_U32_BLOCK,
+                      ABHSF_ERR_INVALID, "unknown delta codec");
+        *out = new abhsf_delta();
+        (*out)->stream = static_cast<hipStream_t>(stream);
+        (*out)->codec = codec;
+    });
+}
+
 abhsf_status abhsf_delta_build_csr(abhsf_delta *d, int64_t m, int64_t n, int64_t nnz, const int64_t *rowptr,
                                    const int32_t *col, const double *val, int32_t on_device)
 {
@@ -581,7 +606,7 @@ abhsf_status abhsf_delta_build_csr(abhsf_delta *d, int64_t m, int64_t n, int64_t
         check_shape(m, n, nnz);
         if (!on_device) {
             ABHSF_REQUIRE(rowptr[m] == nnz, ABHSF_ERR_INVALID, "row pointers do not span nnz");
-            abhsf::delta_build(d->D, m, n, rowptr, col, val, d->stream);
+            abhsf::delta_build(d->D, m, n, rowptr, col, val, d->stream, d->codec);
             return;
         }
         // the encoder runs on the host: bring the device CSR over
@@ -593,7 +618,7 @@ abhsf_status abhsf_delta_build_csr(abhsf_delta *d, int64_t m, int64_t n, int64_t
         if (val && nnz) ABHSF_HIP(hipMemcpyAsync(v.data(), val, v.size() * 8, hipMemcpyDeviceToHost, d->stream));
         ABHSF_HIP(hipStreamSynchronize(d->stream));
         ABHSF_REQUIRE(rp[(size_t)m] == nnz, ABHSF_ERR_INVALID, "row pointers do not span nnz");
-        abhsf::delta_build(d->D, m, n, rp.data(), c.data(), val ? v.data() : nullptr, d->stream);
+        abhsf::delta_build(d->D, m, n, rp.data(), c.data(), val ? v.data() : nullptr, d->stream, d->codec);
     });
 }
 
@@ -614,7 +639,8 @@ abhsf_status abhsf_delta_get_stats(const abhsf_delta *d, abhsf_delta_stats *out)
         out->m = d->D.m;
         out->n = d->D.n;
         out->nnz = d->D.nnz;
-        out->key_bits = d->D.K;
+        out->key_bits = d->D.codec == ABHSF_DELTA_VARINT ? d->D.K : 31;
+        out->codec = d->D.codec;
         out->chunk = abhsf::kDeltaChunk;
         for (int t = 0; t < 4; t++) out->words[t] = d->D.count[t];
         out->stored_bytes = d->D.stored_bytes();
@@ -631,7 +657,9 @@ abhsf_status abhsf_delta_export(const abhsf_delta *d, const char *name, void *ds
         const abhsf::DevBuf *buf = nm == "tags" ? &d->D.tags : nm == "w8" ? &d->D.words[0]
                                    : nm == "w16" ? &d->D.words[1] : nm == "w32" ? &d->D.words[2]
                                    : nm == "w64" ? &d->D.words[3] : nm == "base" ? &d->D.base
-                                   : nm == "start" ? &d->D.start : nm == "val" ? &d->D.val : nullptr;
+                                   : nm == "start" ? &d->D.start : nm == "val" ? &d->D.val
+                                   : nm == "da" ? &d->D.da : nm == "xstart" ? &d->D.xstart
+                                   : nm == "exc" ? &d->D.exc : nullptr;
         ABHSF_REQUIRE(buf, ABHSF_ERR_INVALID, "unknown array " + nm);
         *bytes = buf->bytes();
         if (!dst) return;

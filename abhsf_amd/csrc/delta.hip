@@ -14,6 +14,7 @@
 // shifts plus the row broadcasts (no LDS traffic).
 #include "delta.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <vector>
 
@@ -121,6 +122,90 @@ __global__ __launch_bounds__(256) void k_delta_spmv(const uint32_t *__restrict__
     }
 }
 
+// Inclusive max over the wave of values >= 0 (out-of-row DPP sources read 0).
+__device__ inline uint32_t wave_incl_max_u32(uint32_t v)
+{
+    v = max(v, dpp_u32<DPP_ROW_SHR1>(v));
+    v = max(v, dpp_u32<DPP_ROW_SHR2>(v));
+    v = max(v, dpp_u32<DPP_ROW_SHR4>(v));
+    v = max(v, dpp_u32<DPP_ROW_SHR8>(v));
+    v = max(v, dpp_u32<DPP_ROW_BCAST15, 0xa>(v));
+    v = max(v, dpp_u32<DPP_ROW_BCAST31, 0xc>(v));
+    return v;
+}
+
+// u32 delta codecs: a wavefront decodes 64 elements per step.  Deltas are
+// one coalesced u32 load; escapes (0 at any element but a chunk's first)
+// take their key from the exception list at the chunk's offset plus the
+// escapes before them (ballot + popcount).  Keys: the inclusive 64-bit scan
+// S of the deltas (escapes add 0) plus, per lane, the base of its segment --
+// the chunk carry before the first escape, (exception key - S at the escape)
+// after it, found by a max-scan of escape lane indices and one shuffle.
+// Products are summed over runs of equal rows (segmented DPP scan) and the
+// last lane of a run adds its sum to y atomically: row-major keys give one
+// run per row and step, 8 x 8 cache-block keys one per row and cache block.
+template <int CODEC>
+__global__ __launch_bounds__(256) void k_delta32_spmv(const uint32_t *__restrict__ da,
+                                                      const uint64_t *__restrict__ base,
+                                                      const uint32_t *__restrict__ xstart,
+                                                      const uint64_t *__restrict__ exc,
+                                                      const double *__restrict__ val, int64_t nnz, int64_t nchunk,
+                                                      const double *__restrict__ x, double *y, double alpha)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 4 + wave;
+    if (c >= nchunk) return;
+    uint64_t carry = base[c];
+    uint32_t xo = xstart[c];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll 1
+    for (int step = 0; step < kStepsD; step++) {
+        const int64_t e0 = c * kDeltaChunk + (int64_t)step * kWaveD;
+        if (e0 >= nnz) break;  // wave-uniform
+        const int64_t e = e0 + lane;
+        const bool valid = e < nnz;
+        const uint32_t d = valid ? da[e] : 0u;
+        const bool esc = valid && d == 0u && !(step == 0 && lane == 0);
+        const uint64_t mesc = __ballot(esc);
+        uint64_t ekey = 0;
+        if (esc) ekey = exc[xo + (uint32_t)__popcll(mesc & lt)];
+        xo += (uint32_t)__popcll(mesc);
+        const uint64_t S = wave_incl_scan_u64(esc ? 0ull : (uint64_t)d);
+        uint64_t key = carry + S;
+        if (mesc) {  // wave-uniform: some lane escapes in this step
+            const uint32_t last = wave_incl_max_u32(esc ? (uint32_t)lane + 1u : 0u);
+            const uint64_t b = ekey - S;  // meaningful at escape lanes
+            const uint32_t src = last ? last - 1u : 0u;
+            const uint32_t blo = (uint32_t)__shfl((int)(uint32_t)b, (int)src, kWaveD);
+            const uint32_t bhi = (uint32_t)__shfl((int)(uint32_t)(b >> 32), (int)src, kWaveD);
+            if (last) key = (((uint64_t)bhi << 32) | blo) + S;
+        }
+        carry = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(key >> 32), kWaveD - 1) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, kWaveD - 1);
+        uint64_t row, col;
+        if constexpr (CODEC == kDeltaU32Row) {
+            row = key >> 31;
+            col = key & ((1ull << 31) - 1ull);
+        } else {
+            row = ((key >> 31) << 3) | ((key >> 3) & 7ull);
+            col = (((key >> 6) & ((1ull << 25) - 1ull)) << 3) | (key & 7ull);
+        }
+        if (!valid) row = ~0ull;
+        double p = valid ? alpha * val[e] * x[col] : 0.0;
+        const uint64_t prev = shfl_up_u64(row, 1);
+        const uint32_t head = (lane == 0 || prev != row) ? 1u : 0u;
+        const uint32_t next_head = (uint32_t)__shfl_down((int)head, 1, kWaveD);
+        uint32_t f = head;
+        seg_step<DPP_ROW_SHR1>(p, f);
+        seg_step<DPP_ROW_SHR2>(p, f);
+        seg_step<DPP_ROW_SHR4>(p, f);
+        seg_step<DPP_ROW_SHR8>(p, f);
+        seg_step<DPP_ROW_BCAST15, 0xa>(p, f);
+        seg_step<DPP_ROW_BCAST31, 0xc>(p, f);
+        if (valid && (lane == kWaveD - 1 || next_head)) atomicAdd(&y[row], p);
+    }
+}
+
 __global__ void k_delta_scale(double *y, int64_t m, double beta)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -131,16 +216,85 @@ __global__ void k_delta_scale(double *y, int64_t m, double beta)
 
 uint64_t DeltaStorage::stored_bytes() const
 {
-    uint64_t b = tags.bytes() + base.bytes() + start.bytes() + val.bytes();
+    uint64_t b = tags.bytes() + base.bytes() + start.bytes() + val.bytes() + da.bytes() + xstart.bytes() +
+                 exc.bytes();
     for (int t = 0; t < 4; t++) b += words[t].bytes();
     return b;
 }
 
+namespace {
+
+// Host encoder of the u32 codecs (layout documented in delta.hpp); the CSR
+// has been validated.
+void delta32_encode(DeltaStorage &D, const int64_t *rowptr, const int32_t *col, const double *val, hipStream_t st)
+{
+    const int64_t nnz = D.nnz, m = D.m;
+    std::vector<uint64_t> key((size_t)nnz);
+    std::vector<uint32_t> idx((size_t)nnz);
+    for (int64_t r = 0; r < m; r++)
+        for (int64_t q = rowptr[r]; q < rowptr[r + 1]; q++) {
+            const uint64_t rr = (uint64_t)r, cc = (uint64_t)col[q];
+            key[(size_t)q] = D.codec == kDeltaU32Row ? (rr << 31) + cc
+                                                     : ((rr >> 3) << 31) + ((cc >> 3) << 6) + ((rr & 7) << 3) + (cc & 7);
+            idx[(size_t)q] = (uint32_t)q;
+        }
+    if (D.codec == kDeltaU32Block) {
+        // (row/8, col/8, row%8, col%8) order = ascending key; elements only
+        // move within their band of 8 rows, whose range is the same in CSR
+        // order, so each band is sorted on its own
+        for (int64_t r0 = 0; r0 < m; r0 += 8) {
+            const int64_t q0 = rowptr[r0], q1 = rowptr[std::min<int64_t>(r0 + 8, m)];
+            std::sort(idx.begin() + q0, idx.begin() + q1, [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+        }
+    }
+    std::vector<uint32_t> da((size_t)nnz), xstart((size_t)D.nchunk + 1);
+    std::vector<uint64_t> base((size_t)D.nchunk), exc;
+    std::vector<double> v((size_t)nnz);
+    uint64_t prev = 0;
+    for (int64_t e = 0; e < nnz; e++) {
+        const uint32_t q = idx[(size_t)e];
+        const uint64_t k = key[q], d = k - prev;
+        const bool fits = d < (1ull << 32);
+        if (e % kDeltaChunk == 0) {
+            base[(size_t)(e / kDeltaChunk)] = fits ? prev : k;
+            da[(size_t)e] = fits ? (uint32_t)d : 0u;
+            xstart[(size_t)(e / kDeltaChunk)] = (uint32_t)exc.size();
+        } else if (fits) {
+            da[(size_t)e] = (uint32_t)d;
+        } else {
+            da[(size_t)e] = 0u;
+            exc.push_back(k);
+        }
+        v[(size_t)e] = val ? val[q] : 1.0;
+        prev = k;
+    }
+    xstart[(size_t)D.nchunk] = (uint32_t)exc.size();
+    D.count[2] = (uint64_t)nnz - exc.size();
+    D.count[3] = exc.size();
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+        b.alloc(bytes);
+        if (bytes) ABHSF_HIP(hipMemcpyAsync(b.get(), src, bytes, hipMemcpyHostToDevice, st));
+    };
+    up(D.da, da.data(), da.size() * 4);
+    up(D.base, base.data(), base.size() * 8);
+    up(D.xstart, xstart.data(), xstart.size() * 4);
+    up(D.exc, exc.data(), exc.size() * 8);
+    up(D.val, v.data(), v.size() * 8);
+    ABHSF_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
 void delta_build(DeltaStorage &D, int64_t m, int64_t n, const int64_t *rowptr, const int32_t *col,
-                 const double *val, hipStream_t st)
+                 const double *val, hipStream_t st, int codec)
 {
     const auto t0 = std::chrono::steady_clock::now();
     D = DeltaStorage();
+    ABHSF_REQUIRE(codec == kDeltaVarint || codec == kDeltaU32Row || codec == kDeltaU32Block, ABHSF_ERR_INVALID,
+                  "unknown delta codec");
+    ABHSF_REQUIRE(codec != kDeltaU32Block || n <= (1ll << 28), ABHSF_ERR_UNSUPPORTED,
+                  "the 8 x 8 cache-block key holds column blocks in 25 bits: n must be at most 2^28");
+    D.codec = codec;
     ABHSF_REQUIRE(m >= 1 && n >= 1 && m <= INT32_MAX && n <= INT32_MAX, ABHSF_ERR_INVALID,
                   "matrix dimensions must be in [1, 2^31-1]");
     ABHSF_REQUIRE(rowptr && rowptr[0] == 0, ABHSF_ERR_INVALID, "row pointers must start at 0");
@@ -169,6 +323,7 @@ void delta_build(DeltaStorage &D, int64_t m, int64_t n, const int64_t *rowptr, c
             ABHSF_REQUIRE(cc >= 0 && (int64_t)cc < n, ABHSF_ERR_INVALID, "column out of range");
             ABHSF_REQUIRE(q == rowptr[r] || cc > col[q - 1], ABHSF_ERR_INVALID,
                           "columns not strictly increasing within a row");
+            if (codec != kDeltaVarint) continue;  // validated here, encoded by delta32_encode
             const uint64_t key = ((uint64_t)r << D.K) + (uint64_t)cc;
             if (e % kDeltaChunk == 0) {
                 const int64_t c = e / kDeltaChunk;
@@ -187,6 +342,12 @@ void delta_build(DeltaStorage &D, int64_t m, int64_t n, const int64_t *rowptr, c
             tags[(size_t)(e >> 4)] |= tag << (2 * (e & 15));
             prev = key;
         }
+    }
+    if (codec != kDeltaVarint) {
+        delta32_encode(D, rowptr, col, val, st);
+        D.built = true;
+        D.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return;
     }
     D.count[0] = w8.size();
     D.count[1] = w16.size();
@@ -228,6 +389,18 @@ void delta_spmv(const DeltaStorage &D, double alpha, const double *x, double bet
     if (D.nchunk == 0 || alpha == 0.0) return;
     const int64_t nblk = (D.nchunk + 3) / 4;
     ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many chunks");
+    if (D.codec != kDeltaVarint) {
+        const uint32_t *da = D.da.get<uint32_t>(), *xs = D.xstart.get<uint32_t>();
+        const uint64_t *bs = D.base.get<uint64_t>(), *ex = D.exc.get<uint64_t>();
+        if (D.codec == kDeltaU32Row)
+            hipLaunchKernelGGL(k_delta32_spmv<kDeltaU32Row>, dim3((unsigned)nblk), dim3(256), 0, st, da, bs, xs, ex,
+                               D.val.get<double>(), D.nnz, D.nchunk, x, y, alpha);
+        else
+            hipLaunchKernelGGL(k_delta32_spmv<kDeltaU32Block>, dim3((unsigned)nblk), dim3(256), 0, st, da, bs, xs,
+                               ex, D.val.get<double>(), D.nnz, D.nchunk, x, y, alpha);
+        ABHSF_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_delta_spmv, dim3((unsigned)nblk), dim3(256), 0, st, D.tags.get<uint32_t>(),
                        D.words[0].get<uint8_t>(), D.words[1].get<uint16_t>(), D.words[2].get<uint32_t>(),
                        D.words[3].get<uint64_t>(), D.base.get<uint64_t>(), D.start.get<uint32_t>(),

@@ -18,15 +18,34 @@ namespace abhsf {
 // decoding without looking at other chunks.
 constexpr int kDeltaChunk = 512;
 
+// Codecs (ABHSF_DELTA_* in the C ABI):
+//   kDeltaVarint     the tagged varint words above (spmv/unified/spmv.cpp:859-944);
+//   kDeltaU32Row     u32 deltas of key = (row << 31) + col in row-major order
+//                    (spmv/orig/delta.cpp:124-151);
+//   kDeltaU32Block   u32 deltas of key = ((row/8) << 31) + ((col/8) << 6) +
+//                    ((row%8) << 3) + col%8, elements ordered by (row/8, col/8,
+//                    row%8, col%8): 8 x 8 cache blocks of x and y
+//                    (spmv/unified/spmv.cpp:636-689); needs n <= 2^28.
+// The u32 codecs keep the reference's fixed-width words; where the reference
+// asserts that a delta fits 32 bits, a chunk's first element instead moves
+// its key into the chunk base (delta 0), and any other element stores 0 --
+// never a real delta there, keys strictly increase -- with its key in the
+// exception list exc (element order; chunk c's first exception xstart[c]).
+enum { kDeltaVarint = 0, kDeltaU32Row = 1, kDeltaU32Block = 2 };
+
 struct DeltaStorage {
     int64_t m = 0, n = 0, nnz = 0, nchunk = 0;
     int K = 1;
-    uint64_t count[4] = {0, 0, 0, 0};  // elements per width class
-    DevBuf tags;                       // u32[(nnz + 15) / 16]
-    DevBuf words[4];                   // u8 / u16 / u32 / u64 tagged words
+    int codec = kDeltaVarint;
+    uint64_t count[4] = {0, 0, 0, 0};  // elements per width class (u32 codecs: [2] words, [3] exceptions)
+    DevBuf tags;                       // varint: u32[(nnz + 15) / 16]
+    DevBuf words[4];                   // varint: u8 / u16 / u32 / u64 tagged words
     DevBuf base;                       // u64[nchunk]
-    DevBuf start;                      // u32[nchunk][4]
-    DevBuf val;                        // f64[nnz]
+    DevBuf start;                      // varint: u32[nchunk][4]
+    DevBuf da;                         // u32 codecs: u32[nnz]
+    DevBuf xstart;                     // u32 codecs: u32[nchunk + 1]
+    DevBuf exc;                        // u32 codecs: u64[exceptions]
+    DevBuf val;                        // f64[nnz], in element order
     bool built = false;
     double build_ms = 0.0;
 
@@ -34,9 +53,10 @@ struct DeltaStorage {
 };
 
 // Build from a host CSR (rowptr int64[m+1], col int32[nnz] ascending within
-// rows, val f64[nnz] or null = all ones); throws Error on invalid input.
+// rows, val f64[nnz] or null = all ones) with the given codec; throws Error
+// on invalid input.
 void delta_build(DeltaStorage &D, int64_t m, int64_t n, const int64_t *rowptr, const int32_t *col,
-                 const double *val, hipStream_t st);
+                 const double *val, hipStream_t st, int codec = kDeltaVarint);
 
 // y = alpha*A*x + beta*y (device vectors).
 void delta_spmv(const DeltaStorage &D, double alpha, const double *x, double beta, double *y, hipStream_t st);

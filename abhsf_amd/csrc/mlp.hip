@@ -77,6 +77,45 @@ __global__ __launch_bounds__(kT) void k_mlp_fwd(const float *__restrict__ theta,
     out[b] = o;
 }
 
+// Ensemble inference in one launch: out[b] = mean over the nm members of the
+// forward pass of sample b (thetas: nm parameter vectors of P floats, one
+// after the other).  Each member's parameters are staged in LDS in turn; the
+// members' outputs are summed in FP64 in member order.
+__global__ __launch_bounds__(kT) void k_mlp_ens_fwd(const float *__restrict__ thetas, int nm, int F,
+                                                    const float *__restrict__ X, int64_t B, double *__restrict__ out)
+{
+    __shared__ float sW[H * kMaxF + H + H * H + H + H + 1];
+    const Layout L = layout(F);
+    const int64_t b = (int64_t)blockIdx.x * kT + threadIdx.x;
+    double acc = 0.0;
+    for (int mb = 0; mb < nm; mb++) {
+        __syncthreads();  // the previous member's parameters are no longer read
+        const float *theta = thetas + (int64_t)mb * L.P;
+        for (int64_t i = threadIdx.x; i < L.P; i += blockDim.x) sW[i] = theta[i];
+        __syncthreads();
+        if (b >= B) continue;
+        float h1[H];
+#pragma unroll
+        for (int i = 0; i < H; i++) h1[i] = sW[L.b1 + i];
+        for (int f = 0; f < F; f++) {
+            const float xf = X[b * F + f];
+#pragma unroll
+            for (int i = 0; i < H; i++) h1[i] = fmaf(sW[L.W1 + (int64_t)i * F + f], xf, h1[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < H; i++) h1[i] = h1[i] > 0.f ? h1[i] : 0.f;
+        float o = sW[L.b3];
+        for (int i = 0; i < H; i++) {
+            float a = sW[L.b2 + i];
+#pragma unroll
+            for (int j = 0; j < H; j++) a = fmaf(sW[L.W2 + i * H + j], h1[j], a);
+            o = fmaf(sW[L.W3 + i], a > 0.f ? a : 0.f, o);
+        }
+        acc += (double)o;
+    }
+    if (b < B) out[b] = acc / nm;
+}
+
 // Per-sample deltas: D3 = dL/dout, D2 = dL/dz2, D1 = dL/dz1.
 // wsum_p (device, may be null) replaces wsum when set.
 __global__ __launch_bounds__(kT) void k_mlp_delta(const float *__restrict__ theta, int F, int64_t B,
@@ -246,6 +285,23 @@ extern "C" abhsf_status abhsf_mlp_forward_impl(const float *theta, int32_t F, co
         ABHSF_REQUIRE(F >= 1 && F <= kMaxF, ABHSF_ERR_UNSUPPORTED, "feature count must be in [1, 64]");
         if (B == 0) return ABHSF_OK;
         k_mlp_fwd<<<g(B), kT, 0, static_cast<hipStream_t>(stream)>>>(theta, F, X, B, H1, H2, out);
+        ABHSF_HIP(hipGetLastError());
+        return ABHSF_OK;
+    } catch (const Error &e) {
+        *err = e.what();
+        return e.code;
+    }
+}
+
+extern "C" abhsf_status abhsf_mlp_predict_impl(const float *thetas, int32_t nm, int32_t F, const float *X,
+                                               int64_t B, double *out, void *stream, std::string *err)
+{
+    try {
+        ABHSF_REQUIRE(thetas && X && out && B >= 0, ABHSF_ERR_INVALID, "null argument");
+        ABHSF_REQUIRE(F >= 1 && F <= kMaxF, ABHSF_ERR_UNSUPPORTED, "feature count must be in [1, 64]");
+        ABHSF_REQUIRE(nm >= 1, ABHSF_ERR_INVALID, "at least one ensemble member");
+        if (B == 0) return ABHSF_OK;
+        k_mlp_ens_fwd<<<g(B), kT, 0, static_cast<hipStream_t>(stream)>>>(thetas, nm, F, X, B, out);
         ABHSF_HIP(hipGetLastError());
         return ABHSF_OK;
     } catch (const Error &e) {

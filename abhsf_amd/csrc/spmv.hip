@@ -1081,18 +1081,32 @@ __device__ inline void dense_block_mfma(const VT *__restrict__ D, int s, int64_t
 #endif
 constexpr int kSpmmRows = ABHSF_SPMM_ROWS;
 
+// Two launch shapes: sib == 0, wave w of workgroup b takes strip b * wpb + w
+// for the R right-hand sides at X, Y; sib > 0, the workgroup's sib waves all
+// take strip b, wave w the R-column tile w of the sib * R columns (nrhs in
+// all): the siblings read the same matrix entries together, so the matrix
+// streams from HBM once for all tiles and the tiles' X values share cache
+// lines in the CU's L1.
 template <typename VT, typename BT, typename IT, typename RT, int R, bool COMP>
 __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__restrict__ X, int64_t ldx,
                                                     double *__restrict__ Y, int64_t ldy, int nrhs,
-                                                    double alpha, double beta, int wpb, int64_t nstrip_total)
+                                                    double alpha, double beta, int wpb, int64_t nstrip_total,
+                                                    int sib)
 {
     constexpr int E = kWave / R;  // entries per wave step
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int slot = lane / R, j = lane % R;
-    const bool jok = j < nrhs;
     const int s = A.s, k = A.k;
-    const int64_t I = (int64_t)blockIdx.x * wpb + wave;
+    int64_t I = (int64_t)blockIdx.x * wpb + wave;
+    if (sib) {
+        I = blockIdx.x;
+        X += (int64_t)wave * R;
+        Y += (int64_t)wave * R;
+        nrhs -= wave * R;
+        if (nrhs <= 0) return;  // whole wave leaves; no workgroup barriers below
+    }
+    const bool jok = j < nrhs;
     if (I >= nstrip_total) return;  // whole wave leaves; no workgroup barriers below
 
     const size_t per_wave = spmm_wave_lds(s, R, COMP);
@@ -1350,19 +1364,26 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
     }
 }
 
+// nrhs <= R: one tile, wpb strips per workgroup; nrhs > R (at most
+// kSpmmSiblings * R): sibling waves, one strip per workgroup, one R-column
+// tile per wave.
+constexpr int kSpmmSiblings = 4;
+
 template <typename VT, typename BT, typename IT, typename RT, int R, bool COMP>
 void launch_t(const DevMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy, int nrhs, double alpha,
               double beta, hipStream_t st)
 {
     const size_t per_wave = spmm_wave_lds(A.s, R, COMP);
-    int wpb = (int)(65536 / per_wave);
+    const int sib = nrhs > R ? (nrhs + R - 1) / R : 0;
+    ABHSF_REQUIRE(sib <= kSpmmSiblings, ABHSF_ERR_UNSUPPORTED, "more column tiles than sibling waves");
+    int wpb = sib ? sib : (int)(65536 / per_wave);
     if (wpb > 4) wpb = 4;
-    ABHSF_REQUIRE(wpb >= 1, ABHSF_ERR_UNSUPPORTED, "output tile does not fit in LDS");
-    const int64_t nblk = (A.M + wpb - 1) / wpb;
+    ABHSF_REQUIRE(wpb >= 1 && per_wave * wpb <= 65536, ABHSF_ERR_UNSUPPORTED, "output tile does not fit in LDS");
+    const int64_t nblk = sib ? A.M : (A.M + wpb - 1) / wpb;
     ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many strips");
     if (A.M == 0) return;
     hipLaunchKernelGGL((k_spmm_strip<VT, BT, IT, RT, R, COMP>), dim3((unsigned)nblk), dim3(64 * wpb),
-                       per_wave * wpb, st, A, X, ldx, Y, ldy, nrhs, alpha, beta, wpb, A.M);
+                       per_wave * wpb, st, A, X, ldx, Y, ldy, nrhs, alpha, beta, wpb, A.M, sib);
     ABHSF_HIP(hipGetLastError());
 }
 
@@ -1530,11 +1551,15 @@ void spmm(const SpmvLaunch &L, const DevMat &A, int nrhs, double alpha, const do
     // larger s x R accumulator in LDS per wave and lose more occupancy than
     // the extra pass over the matrix costs (stencil27, s = 64: k = 32 as one
     // tile 12.2 ms, as 2 x 16 4.3 ms each, as 4 x 8 1.5 ms each)
+    // up to kSpmmSiblings tiles per launch go to sibling waves of one strip
+    // (the matrix is read once for all of them)
     const int rmax = std::min(max_rhs_tile(A.s, comp), L.has_dense && !comp ? 16 : 8);
-    for (int j0 = 0; j0 < nrhs; j0 += rmax) {
-        const int nr = nrhs - j0 < rmax ? nrhs - j0 : rmax;
+    const int sibmax = std::max(1, std::min(kSpmmSiblings, (int)(65536 / spmm_wave_lds(A.s, rmax, comp))));
+    const int span = rmax * sibmax;
+    for (int j0 = 0; j0 < nrhs; j0 += span) {
+        const int nr = nrhs - j0 < span ? nrhs - j0 : span;
         int R = 1;
-        while (R < nr) R <<= 1;
+        while (R < nr && R < rmax) R <<= 1;
         if (comp) launch_v<float, true>(L, A, R, X + j0, ldx, Y + j0, ldy, nr, alpha, beta, st);
         else launch_v<double, false>(L, A, R, X + j0, ldx, Y + j0, ldy, nr, alpha, beta, st);
     }

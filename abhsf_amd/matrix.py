@@ -249,16 +249,22 @@ class AbhsfBatch:
             pass
 
 
+_CODEC = {"varint": capi.DELTA_VARINT, "u32_row": capi.DELTA_U32_ROW, "u32_block": capi.DELTA_U32_BLOCK}
+
+
 class DeltaMatrix:
     """Delta-coded coordinate storage resident in HBM: the reference's
-    delta_matrix (spmv/unified/spmv.cpp:628-1033) with its tagged varint
-    words laid out for wavefront decode (see include/abhsf_c.h)."""
+    delta_matrix family (spmv/unified/spmv.cpp:628-1033, spmv/orig/delta.cpp:
+    118-160) laid out for wavefront decode (see include/abhsf_c.h).  codec:
+    "varint" (tagged varint words of row-major keys), "u32_row" (u32 deltas of
+    row << 31 | col) or "u32_block" (u32 deltas of 8 x 8 cache-block keys)."""
 
-    def __init__(self, stream=None):
+    def __init__(self, stream=None, codec: str = "varint"):
         self._stream = stream
+        self.codec = codec
         self._h = C.c_void_p(0)
-        capi.call("abhsf_delta_create", stream.handle if isinstance(stream, hip.Stream) else stream,
-                  C.byref(self._h))
+        capi.call("abhsf_delta_create_codec", stream.handle if isinstance(stream, hip.Stream) else stream,
+                  _CODEC[codec], C.byref(self._h))
 
     def from_elements(self, elements, props: matrix_properties) -> None:
         """Build from (rows, cols, vals) arrays or (row, col, val) tuples; a
@@ -305,7 +311,8 @@ class DeltaMatrix:
 
     def export(self, name: str) -> np.ndarray:
         dt = {"tags": np.uint32, "w8": np.uint8, "w16": np.uint16, "w32": np.uint32, "w64": np.uint64,
-              "base": np.uint64, "start": np.uint32, "val": np.float64}[name]
+              "base": np.uint64, "start": np.uint32, "val": np.float64, "da": np.uint32, "xstart": np.uint32,
+              "exc": np.uint64}[name]
         nbytes = C.c_size_t(0)
         capi.call("abhsf_delta_export", self._h, name.encode(), None, 0, C.byref(nbytes))
         out = np.empty(nbytes.value // np.dtype(dt).itemsize, dtype=dt)

@@ -41,9 +41,26 @@ import numpy as np
 from . import hip, synth, tuner
 
 
+# Training matrices of the same configuration (family and parameters, another
+# seed) as a held-out or benchmark matrix: left out, so that held-out regret
+# measures unseen configurations.
+EXCLUDED_SAME_CONFIG = {
+    "mix_1048576_0.2_0.3": "block_mix n=2^20 (0.2, 0.3) = held-out mix_2^20_a",
+    "mix_4194304_0.2_0.3": "block_mix n=2^22 (0.2, 0.3) = held-out block_mix_2^22 and the benchmark matrix",
+    "rmat_22_16": "R-MAT scale 22 edge factor 16 = held-out rmat_22_16 and the benchmark matrix",
+}
+
+
+def family(name: str) -> str:
+    return {"lap": "laplacian", "stencil": "stencil27", "mix": "block_mix", "block": "block_mix", "rmat": "rmat",
+            "banded": "banded", "rblk": "random_blocks"}[name.split("_")[0]]
+
+
 def corpus(split: str):
     """(name, generator) pairs."""
     if split == "train":
+        return [(nm, g) for nm, g in corpus("all_train") if nm not in EXCLUDED_SAME_CONFIG]
+    if split == "all_train":
         out = [(f"lap_{g}", (lambda g=g: synth.laplacian2d(g))) for g in (256, 450, 700, 1500, 2400)]
         out += [(f"stencil_{g}", (lambda g=g: synth.stencil27(g))) for g in (40, 64, 80, 128, 150)]
         for n in (1 << 17, 1 << 19, 1 << 21):
@@ -83,13 +100,15 @@ def collect(split, stream, log):
         t0 = time.perf_counter()
         A = gen()
         d_rp, d_c = hip.DeviceArray.from_numpy(A.rowptr), hip.DeviceArray.from_numpy(A.col)
+        tf = time.perf_counter()
         Xc, per_s = tuner.matrix_features(A, d_rp, d_c, stream)
+        features_ms = (time.perf_counter() - tf) * 1e3
         tm = tuner.sweep(A, stream)
         for i, cand in enumerate(tuner.candidates()):
             X.append(Xc[i])
             y.append(math.log(tm[cand] / A.nnz))
         names.append(name)
-        times.append({f"{s}/{v}": tm[(s, v)] for s, v in tm} | {"nnz": A.nnz,
+        times.append({f"{s}/{v}": tm[(s, v)] for s, v in tm} | {"nnz": A.nnz, "features_ms": features_ms,
                      "storage_opt": list(tuner.pick_storage_optimal(per_s))})
         best = min(tm, key=tm.get)
         log(f"{split} {name}: nnz={A.nnz} best={best} {tm[best]:.4f} ms "
@@ -207,16 +226,47 @@ def main_batch(args, log):
         f"mean regret {report['mean_regret_pct']}")
 
 
+def cross_validate(Xtr, ytr, ntr, nc, grid, folds=5, members=5, seed=0, log=print):
+    """k-fold cross-validation by matrix over the training set: for every
+    (steps, lr) of the grid, the mean regret of the held-in model's pick on
+    each held-out fold's matrices (labels ytr are log times per entry, so
+    time ratios within a matrix are exp of label differences).  Returns
+    {(steps, lr): mean regret %} and the best setting."""
+    nmat = len(ntr)
+    order = np.random.default_rng(seed).permutation(nmat)
+    fold_of = np.empty(nmat, dtype=np.int64)
+    fold_of[order] = np.arange(nmat) % folds
+    Y = ytr.reshape(nmat, nc)
+    X = Xtr.reshape(nmat, nc, -1)
+    res = {}
+    for steps, lr in grid:
+        regrets = []
+        for f in range(folds):
+            tr, te = np.flatnonzero(fold_of != f), np.flatnonzero(fold_of == f)
+            model, _ = tuner.train(X[tr].reshape(-1, X.shape[-1]), Y[tr].reshape(-1), steps=steps, lr=lr,
+                                   members=members)
+            for i in te:
+                k = int(np.argmin(model.predict(X[i])))
+                regrets.append((math.exp(Y[i, k] - Y[i].min()) - 1.0) * 100.0)
+        res[(steps, lr)] = float(np.mean(regrets))
+        log(f"cv steps={steps} lr={lr}: mean regret {res[(steps, lr)]:.2f}%")
+    return res, min(res, key=res.get)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    # latency defaults chosen by 5-fold cross-validation over the training
-    # matrices (grid steps {300 .. 6000} x lr {1e-3, 3e-3, 1e-2}, mean
-    # regret); the batch set is 8x larger and keeps the longer schedule
+    # latency defaults: the choice of a 5-fold cross-validation over the
+    # training matrices (--cv reruns it); the batch set is 8x larger and
+    # keeps the longer schedule
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--lr", type=float, default=None)
     ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--mode", choices=["latency", "batch"], default="latency")
+    ap.add_argument("--cv", action="store_true",
+                    help="latency: choose steps and lr by 5-fold cross-validation over the training matrices")
+    ap.add_argument("--from-dataset", default=None,
+                    help="latency: retrain from a saved tuner_dataset.npz (its labels) instead of sweeping")
     args = ap.parse_args(argv)
     if args.steps is None:
         args.steps = 4000 if args.mode == "batch" else 700
@@ -231,20 +281,43 @@ def main(argv=None):
         return main_batch(args, log)
 
     stream = hip.Stream()
-    Xtr, ytr, ntr, ttr = collect("train", stream, log)
-    Xte, yte, nte, tte = collect("test", stream, log)
-    np.savez(os.path.join(args.out, "tuner_dataset.npz"), Xtr=Xtr, ytr=ytr, Xte=Xte, yte=yte)
-    model, losses = tuner.train(Xtr, ytr, steps=args.steps, lr=args.lr, members=args.members)
-    log(f"trained: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {model.meta['train_s']:.1f} s")
-    model.save(os.path.join(args.out, "tuner_model.json"))
-
     cands = tuner.candidates()
     nc = len(cands)
-    report = {"train_loss_first": float(losses[0]), "train_loss_last": float(losses[-1]), "matrices": []}
+    if args.from_dataset:  # labels already measured (same kernels): retrain and report only
+        d = np.load(args.from_dataset, allow_pickle=False)
+        Xtr, ytr, Xte, yte = d["Xtr"], d["ytr"], d["Xte"], d["yte"]
+        ntr, nte = [str(v) for v in d["train_names"]], [str(v) for v in d["test_names"]]
+        tte = json.loads(str(d["test_times"]))
+        keep = np.repeat([nm not in EXCLUDED_SAME_CONFIG for nm in ntr], nc)
+        Xtr, ytr, ntr = Xtr[keep], ytr[keep], [nm for nm in ntr if nm not in EXCLUDED_SAME_CONFIG]
+    else:
+        Xtr, ytr, ntr, ttr = collect("train", stream, log)
+        Xte, yte, nte, tte = collect("test", stream, log)
+        np.savez(os.path.join(args.out, "tuner_dataset.npz"), Xtr=Xtr, ytr=ytr, Xte=Xte, yte=yte,
+                 train_names=np.array(ntr), test_names=np.array(nte), test_times=np.array(json.dumps(tte)),
+                 candidates=np.array([f"{s}/{v}" for s, v in cands]))
+    cv = None
+    if args.cv:
+        grid = [(st, lr) for st in (300, 700, 1500, 3000) for lr in (3e-3, 1e-2)]
+        cv, (args.steps, args.lr) = cross_validate(Xtr, ytr, ntr, nc, grid, members=args.members, log=log)
+        log(f"cv choice: steps={args.steps} lr={args.lr}")
+    model, losses = tuner.train(Xtr, ytr, steps=args.steps, lr=args.lr, members=args.members)
+    log(f"trained: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {model.meta['train_s']:.1f} s")
+    model.meta["train_matrices"] = list(ntr)
+    model.save(os.path.join(args.out, "tuner_model.json"))
+    model = tuner.Model.load(os.path.join(args.out, "tuner_model.json"))
+
+    report = {"train_loss_first": float(losses[0]), "train_loss_last": float(losses[-1]),
+              "schedule": {"steps": args.steps, "lr": args.lr, "members": args.members},
+              "cv_mean_regret_pct": None if cv is None else {f"{k[0]}/{k[1]}": v for k, v in cv.items()},
+              "train_matrices": list(ntr), "excluded_same_config": EXCLUDED_SAME_CONFIG,
+              "candidates": [f"{s}/{v}" for s, v in cands], "matrices": []}
     agg = {"mlp": [], "plain_s64": [], "storage_opt": []}
     for i, name in enumerate(nte):
         tm = {(s, v): tte[i][f"{s}/{v}"] for s, v in cands}
+        tp = time.perf_counter()
         pred = model.predict(Xte[i * nc:(i + 1) * nc])
+        predict_ms = (time.perf_counter() - tp) * 1e3
         pk = cands[int(np.argmin(pred))]
         so = tuple(tte[i]["storage_opt"])
         row = {
@@ -253,6 +326,7 @@ def main(argv=None):
             "regret_mlp": tuner.regret(tm, pk), "regret_plain_s64": tuner.regret(tm, (64, "balanced")),
             "regret_storage_opt": tuner.regret(tm, so), "plain_s64_ms": tm[(64, "balanced")],
             "speedup_vs_plain": tm[(64, "balanced")] / tm[pk],
+            "family": family(name), "tuning_ms": {"features": tte[i].get("features_ms"), "predict": predict_ms},
         }
         report["matrices"].append(row)
         agg["mlp"].append(row["regret_mlp"])
@@ -262,6 +336,12 @@ def main(argv=None):
             f"| storage-opt {so} {row['regret_storage_opt']:.1f}%")
     report["mean_regret"] = {k: float(np.mean(v)) for k, v in agg.items()}
     report["max_regret"] = {k: float(np.max(v)) for k, v in agg.items()}
+    fam = {}
+    for row in report["matrices"]:
+        f = fam.setdefault(row["family"], {"mlp": [], "plain_s64": []})
+        f["mlp"].append(row["regret_mlp"])
+        f["plain_s64"].append(row["regret_plain_s64"])
+    report["mean_regret_by_family"] = {k: {m: float(np.mean(v)) for m, v in d.items()} for k, d in fam.items()}
     json.dump(report, open(os.path.join(args.out, "tuner_report.json"), "w"), indent=1)
     log(f"mean regret: {report['mean_regret']}")
 

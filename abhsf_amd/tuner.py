@@ -39,6 +39,9 @@ NAME = "mlp-2x64"
 S_CANDIDATES = (8, 16, 32, 64, 128, 256)
 VARIANTS = ("strip", "balanced", "panel", "tiled")
 DELTA = (0, "delta")  # the delta-coded coordinate format (no block size), one more candidate
+# the delta formats: candidate variant -> DeltaMatrix codec (spmv/unified/spmv.cpp:628-1033,
+# spmv/orig/delta.cpp:118-160)
+DELTA_CODECS = {"delta": "varint", "delta_u32_row": "u32_row", "delta_u32_block": "u32_block"}
 HIDDEN = 64
 _HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(_HERE, "tuner_model.json")          # single-matrix SpMV latency
@@ -143,7 +146,7 @@ def tile_fraction(rowptr, col, m, s, stride):
         return 0.0
     idx = np.repeat(e0 - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt) + np.arange(tot, dtype=np.int64)
     sid = np.repeat(np.arange(strips.size, dtype=np.int64), cnt)
-    key = (sid << np.int64(32)) | (np.asarray(col, dtype=np.int64)[idx] >> np.int64(k))
+    key = (sid << np.int64(32)) | (col[idx].astype(np.int64) >> np.int64(k))
     u, z = np.unique(key, return_counts=True)
     mixed = np.zeros(strips.size, dtype=bool)
     mixed[(u >> np.int64(32))[z > coo_max_z(s)]] = True
@@ -186,32 +189,131 @@ def feature_vector(g, sf, variant):
         sf["log2_s"], sf["log2_blk_per_entry"], *sf["blk_frac"], *sf["ent_frac"],
         sf["log2_bytes_per_entry"], sf["strip_cv"], sf["log2_max_strip"], sf["heavy_frac"], sf["log2_M"],
         sf["tile_frac"], 1.0 if variant == "balanced" else 0.0, 1.0 if variant == "panel" else 0.0,
-        1.0 if variant == "delta" else 0.0, 1.0 if variant == "tiled" else 0.0,
+        1.0 if variant in DELTA_CODECS else 0.0, 1.0 if variant == "tiled" else 0.0,
     ], dtype=np.float64)
 
 
 NFEAT = 6 + 20  # 6 global + 20 per candidate (fractions are 4 values each)
 
 
-def delta_features(rowptr, col, n):
-    """Per-candidate features of the delta format, exact from the CSR: the
-    fractions of elements in 1/2/4/8-byte words (in the scheme-fraction
-    slots), stored bytes per entry, log2 of the chunk count."""
-    nnz = int(rowptr[-1])
+def _band_sample(rowptr, col, limit=SAMPLE_NNZ, band=64):
+    """Elements of a sample of bands of `band` rows (band-aligned), drawn by
+    systematic sampling over element positions (a band is taken when one of
+    the evenly spaced sample points falls into it, so with probability
+    pi = min(1, entries / step)), about `limit` elements in all.  Returns
+    rows, cols (uint64), the element weights 1 / pi (Horvitz-Thompson: sums
+    of weights estimate totals without bias), each element's predecessor in
+    the whole matrix (row and column; -1 before the first) and the band of
+    every element.  Small matrices: every element, weight 1."""
     m = rowptr.size - 1
-    K = max(1, _ceil_log2(n))
-    rows = np.repeat(np.arange(m, dtype=np.uint64), np.diff(rowptr).astype(np.int64))
-    keys = (rows << np.uint64(K)) + np.asarray(col[:nnz], dtype=np.uint64)
-    d = np.diff(keys, prepend=np.uint64(0))
-    cls = [int(np.count_nonzero(d < (1 << 6))), int(np.count_nonzero((d >= (1 << 6)) & (d < (1 << 14)))),
-           int(np.count_nonzero((d >= (1 << 14)) & (d < (1 << 30)))), int(np.count_nonzero(d >= (1 << 30)))]
-    chunks = -(-nnz // 512)
-    stored = 8 * nnz + cls[0] + 2 * cls[1] + 4 * cls[2] + 8 * cls[3] + 4 * -(-nnz // 16) + 24 * chunks
+    nnz = int(rowptr[-1])
+    nb = -(-m // band)
+    bptr = rowptr[np.minimum(np.arange(nb + 1, dtype=np.int64) * band, m)]
+    size = np.diff(bptr).astype(np.float64)
+    if nnz <= limit:
+        take = np.arange(nb, dtype=np.int64)
+        pi = np.ones(nb)
+    else:
+        # sampling measure: entries plus rows (at the mean row length), so
+        # that per-row events (row gaps, escapes) in sparse bands are sampled
+        # as well as per-entry ones
+        rowsz = np.diff(np.minimum(np.arange(nb + 1, dtype=np.int64) * band, m)).astype(np.float64)
+        meas = size + rowsz * (nnz / max(m, 1))
+        mptr = np.concatenate([[0.0], np.cumsum(meas)])
+        lo, hi = 1.0, float(mptr[-1])  # step with about `limit` expected elements
+        for _ in range(40):
+            step = 0.5 * (lo + hi)
+            if float(np.sum(size * np.minimum(1.0, meas / step))) > limit:
+                lo = step
+            else:
+                hi = step
+        step = hi
+        pts = (np.arange(int(mptr[-1] // step) + 1, dtype=np.float64) + 0.5) * step
+        take = np.unique(np.searchsorted(mptr, pts[pts < mptr[-1]], side="right") - 1)
+        pi = np.minimum(1.0, meas[take] / step)
+    e0 = bptr[take]
+    cnt = (bptr[take + 1] - e0).astype(np.int64)
+    tot = int(cnt.sum())
+    first = np.concatenate([[0], np.cumsum(cnt)[:-1]]) if take.size else np.zeros(0, dtype=np.int64)
+    idx = np.repeat(e0 - first, cnt) + np.arange(tot, dtype=np.int64)
+    rows = (np.searchsorted(rowptr, idx, side="right") - 1).astype(np.int64)
+    prev = idx - 1
+    prow = np.where(prev >= 0, np.searchsorted(rowptr, np.maximum(prev, 0), side="right") - 1, -1)
+    pcol = np.where(prev >= 0, col[np.maximum(prev, 0)], -1).astype(np.int64)
+    w = np.repeat(1.0 / np.maximum(pi, 1e-12), cnt)
+    return (rows.astype(np.uint64), col[idx].astype(np.uint64), w, prow, pcol, np.repeat(take, cnt), idx)
+
+
+def delta_features(rowptr, col, n, codec="varint", sample=None):
+    """Per-candidate features of a delta format from the CSR, the element
+    statistics estimated on a sample of row bands (_band_sample, unbiased):
+    the fractions of elements stored in 1/2/4/8-byte words (in the
+    scheme-fraction slots; the u32 codecs use the 4-byte slot for their words
+    and the 8-byte slot for exceptions), stored bytes per entry, log2 of the
+    runs of equal rows per entry (the SpMV makes one atomic update per run;
+    in the blocks-per-entry slot), log2 of the chunk count."""
+    nnz = int(rowptr[-1])
     tot = max(nnz, 1)
+    chunks = -(-nnz // 512)
+    rows, c, w, prow, pcol, _, idx = sample if sample is not None else _band_sample(rowptr, col)
+    nonempty = int(np.count_nonzero(np.diff(rowptr)))
+    has_prev = prow >= 0
+    pr = np.maximum(prow, 0).astype(np.uint64)
+    pc = np.maximum(pcol, 0).astype(np.uint64)
+    if codec == "varint":
+        K = np.uint64(max(1, _ceil_log2(n)))
+        d = ((rows << K) + c) - np.where(has_prev, (pr << K) + pc, np.uint64(0))
+        cls = [float(np.sum(w[d < (1 << 6)])), float(np.sum(w[(d >= (1 << 6)) & (d < (1 << 14))])),
+               float(np.sum(w[(d >= (1 << 14)) & (d < (1 << 30))])), float(np.sum(w[d >= (1 << 30)]))]
+        stored = 8 * nnz + cls[0] + 2 * cls[1] + 4 * cls[2] + 8 * cls[3] + 4 * -(-nnz // 16) + 24 * chunks
+        runs = nonempty
+    else:
+        chunk_first = idx % 512 == 0  # a chunk's first element never escapes (its position in key order:
+        # the u32_row order is the CSR order; the cache-block order keeps each 8-row band's positions)
+        if codec == "u32_row":
+            d = ((rows << np.uint64(31)) + c) - np.where(has_prev, (pr << np.uint64(31)) + pc, np.uint64(0))
+            nexc = float(np.sum(w[(d >= np.uint64(1 << 32)) & ~chunk_first]))
+            runs = nonempty
+        else:
+            # cache-block order: an element escapes only as the first (smallest
+            # rest key) of an 8-row band following the previous nonempty band's
+            # last (largest rest key) by 2^32 or more; the band's first key in
+            # key order is taken at the band's first CSR position
+            rest = lambda r, cc: ((cc >> np.uint64(3)) << np.uint64(6)) + ((r & np.uint64(7)) << np.uint64(3)) + \
+                (cc & np.uint64(7))
+            band8 = rows >> np.uint64(3)
+            nexc = 0.0
+            if rows.size:
+                bstart = np.flatnonzero(np.diff(band8, prepend=~np.uint64(0)) != 0)
+                rs = rest(rows, c)
+                kfirst = (band8[bstart] << np.uint64(31)) + np.minimum.reduceat(rs, bstart)
+                # the previous nonempty 8-row band in the whole matrix: its last key
+                r0 = (band8[bstart] << np.uint64(3)).astype(np.int64)
+                e0 = rowptr[r0]
+                has = e0 > 0
+                lr = np.searchsorted(rowptr, np.maximum(e0 - 1, 0), side="right") - 1
+                pb = (np.maximum(lr, 0) >> 3).astype(np.int64)
+                q0, q1 = rowptr[pb * 8], rowptr[np.minimum(pb * 8 + 8, rowptr.size - 1)]
+                klast = np.zeros(bstart.size, dtype=np.uint64)
+                for i in np.flatnonzero(has):  # one previous band per sampled 8-row band
+                    rr = np.repeat(np.arange(pb[i] * 8, min(pb[i] * 8 + 8, rowptr.size - 1)),
+                                   np.diff(rowptr[pb[i] * 8:min(pb[i] * 8 + 8, rowptr.size - 1) + 1]))
+                    cc = col[q0[i]:q1[i]].astype(np.uint64)
+                    klast[i] = (np.uint64(pb[i]) << np.uint64(31)) + np.max(rest(rr.astype(np.uint64), cc))
+                esc = has & ((kfirst - klast) >= np.uint64(1 << 32)) & ~chunk_first[bstart]
+                nexc = float(np.sum(w[bstart][esc]))
+            # runs: distinct (row, col / 8) pairs
+            same = np.zeros(rows.size, dtype=bool)
+            if rows.size > 1:
+                same[1:] = (rows[1:] == rows[:-1]) & ((c[1:] >> np.uint64(3)) == (c[:-1] >> np.uint64(3)))
+            runs = float(np.sum(w[~same]))
+        cls = [0.0, 0.0, nnz - nexc, nexc]
+        stored = 12 * nnz + 8 * nexc + 8 * chunks + 4 * (chunks + 1)
     return {
-        "log2_s": 0.0, "log2_blk_per_entry": 0.0, "blk_frac": [0.0] * 4, "ent_frac": [c / tot for c in cls],
-        "log2_bytes_per_entry": math.log2(max(stored, 1) / tot), "strip_cv": 0.0, "log2_max_strip": 0.0,
-        "heavy_frac": 0.0, "log2_M": math.log2(max(chunks, 1)), "tile_frac": 0.0, "stored_bytes": stored,
+        "log2_s": 0.0, "log2_blk_per_entry": math.log2(max(runs, 1) / tot), "blk_frac": [0.0] * 4,
+        "ent_frac": [v / tot for v in cls], "log2_bytes_per_entry": math.log2(max(stored, 1) / tot),
+        "strip_cv": 0.0, "log2_max_strip": 0.0, "heavy_frac": 0.0, "log2_M": math.log2(max(chunks, 1)),
+        "tile_frac": 0.0, "stored_bytes": stored,
     }
 
 
@@ -222,10 +324,11 @@ def delta_features(rowptr, col, n):
 BATCH_VARIANTS = ("balanced", "tiled")
 
 
-def candidates(variants=VARIANTS + ("delta",)):
-    """(s, variant) of the ABHSF configurations, then DELTA when allowed."""
+def candidates(variants=VARIANTS + tuple(DELTA_CODECS)):
+    """(s, variant) of the ABHSF configurations, then the delta formats
+    (block size 0) that are allowed."""
     out = [(s, v) for s in S_CANDIDATES for v in VARIANTS if v in variants]
-    return out + ([DELTA] if "delta" in variants else [])
+    return out + [(0, v) for v in DELTA_CODECS if v in variants]
 
 
 def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
@@ -241,8 +344,9 @@ def matrix_features(A, d_rowptr=None, d_col=None, stream=None):
             hist, nblk, ns = block_hist(A.rowptr, A.col, A.m, A.n, s, stride, False)
         per_s[s] = size_features(A.rowptr, A.m, A.n, s, hist, nblk, ns,
                                  tile_fraction(A.rowptr, A.col, A.m, s, stride))
+    sample = _band_sample(A.rowptr, A.col)  # shared by the delta formats' features
     for s, v in candidates():
-        sf = per_s[s] if v != "delta" else delta_features(A.rowptr, A.col, A.n)
+        sf = per_s[s] if v not in DELTA_CODECS else delta_features(A.rowptr, A.col, A.n, DELTA_CODECS[v], sample)
         rows.append(feature_vector(g, sf, v))
     return np.array(rows), per_s
 
@@ -253,8 +357,8 @@ def build_candidate(A, s, variant, d_rp, d_c, d_v, stream):
     """The matrix object of one candidate (ABHSF at (s, variant), or the
     delta format), built from a device CSR."""
     from .matrix import AbhsfMatrix, DeltaMatrix
-    if variant == "delta":
-        M = DeltaMatrix(stream=stream)
+    if variant in DELTA_CODECS:
+        M = DeltaMatrix(stream=stream, codec=DELTA_CODECS[variant])
     else:
         M = AbhsfMatrix(block_size=s, variant=variant, stream=stream)
     M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
@@ -376,21 +480,18 @@ class Model:
         return ((X - self.mu) / self.sd).astype(np.float32)
 
     def predict(self, X):
-        """log(ms per entry) predictions, computed by the GPU forward kernel
-        (mean over the ensemble members)."""
+        """log(ms per entry) predictions: the mean over the ensemble members,
+        all members and candidates in one launch of the GPU inference kernel
+        (abhsf_mlp_predict), one copy back."""
         Xn = self.normalize(X)
         B, F = Xn.shape
-        dx = hip.DeviceArray.from_numpy(Xn)
-        h1, h2, out = hip.DeviceArray((B, HIDDEN), np.float32), hip.DeviceArray((B, HIDDEN), np.float32), \
-            hip.DeviceArray((B,), np.float32)
-        acc = np.zeros(B)
         members = self.theta.reshape(-1, self.theta.shape[-1])
-        for th in members:
-            dt = hip.DeviceArray.from_numpy(np.ascontiguousarray(th))
-            capi.call("abhsf_mlp_forward", dt.ptr, F, dx.ptr, B, h1.ptr, h2.ptr, out.ptr, None)
-            hip.synchronize()
-            acc += out.numpy().astype(np.float64)
-        return acc / len(members) + self.ymu
+        if getattr(self, "_dtheta", None) is None:  # parameters stay on the device
+            self._dtheta = hip.DeviceArray.from_numpy(np.ascontiguousarray(members))
+        dx = hip.DeviceArray.from_numpy(Xn)
+        out = hip.DeviceArray((B,), np.float64)
+        capi.call("abhsf_mlp_predict", self._dtheta.ptr, members.shape[0], F, dx.ptr, B, out.ptr, None)
+        return out.numpy() + self.ymu
 
 
 def train(X, y, w=None, steps=700, lr=1e-2, seed=0, mu=None, sd=None, members=1):
@@ -428,7 +529,7 @@ def pick(model, Xc):
     """Candidate with the smallest predicted time among the variants the model
     was trained on (Xc: feature rows of every candidate), and the
     predictions."""
-    allowed = model.meta.get("variants", VARIANTS + ("delta",))
+    allowed = model.meta.get("variants", VARIANTS + tuple(DELTA_CODECS))
     pred = model.predict(Xc)
     cands = candidates()
     ok = [i for i, (_, v) in enumerate(cands) if v in allowed]

@@ -141,8 +141,8 @@ def csr_spmv_numpy(A, x):
 
 
 def measure_candidates(A, cands, args, hip, stream, check=True):
-    """Build and time each (s, variant) -- (0, "delta") is the delta-coded
-    format; returns per-candidate results."""
+    """Build and time each (s, variant) -- block size 0 marks the delta-coded
+    formats (tuner.DELTA_CODECS); returns per-candidate results."""
     from abhsf_amd import tuner
     x = np.random.default_rng(0).uniform(-1.0, 1.0, A.n)
     dx = hip.DeviceArray.from_numpy(x)
@@ -155,7 +155,7 @@ def measure_candidates(A, cands, args, hip, stream, check=True):
     for s, v in cands:
         M = tuner.build_candidate(A, s, v, d_rp, d_c, d_v, stream)
         st = M.stats()
-        if v == "delta":  # same keys as the ABHSF statistics
+        if v in tuner.DELTA_CODECS:  # same keys as the ABHSF statistics
             st = {"stored_bytes": st["stored_bytes"], "index_bytes": st["stored_bytes"] - 8 * A.nnz,
                   "blocks": {"words_1_2_4_8": list(st["words"].values())}, "build_ms": st["build_ms"]}
         err = None
@@ -192,7 +192,7 @@ def tuned_pick(A, allowed_s, hip):
     Xc, _ = tuner.matrix_features(A)
     pred = model.predict(Xc)
     cands = tuner.candidates()
-    ok = [i for i, c in enumerate(cands) if c[0] in allowed_s or c == tuner.DELTA]
+    ok = [i for i, c in enumerate(cands) if c[0] in allowed_s or c[0] == 0]
     best = min(ok, key=lambda i: pred[i])
     return cands[best], tuner.NAME
 
@@ -231,6 +231,16 @@ def measured_traffic(key):
                                        f"{rec['write_factor']} at sources {src} (profiles/pmc_traffic.json)")
 
 
+def kernel_kind(name):
+    """Which launch a dispatch belongs to in the counter summaries."""
+    for pat, kind in (("k_spmv_strip", "spmv"), ("k_delta_spmv", "spmv"), ("k_delta32_spmv", "spmv"),
+                      ("k_spmv_fixup", "fixup"), ("k_spmm_strip", "spmm"), ("k_batch_spmv", "batch"),
+                      ("k_batch_fixup", "batch_fixup"), ("k_stream_copy", "copy")):
+        if pat in name:
+            return kind
+    return None
+
+
 def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
     """Per-dispatch HBM traffic of the SpMV kernels from rocprofv3 --pmc CSVs.
 
@@ -248,9 +258,7 @@ def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
     def per_kernel(path):
         acc = defaultdict(lambda: defaultdict(list))
         for row in csv.DictReader(open(path)):
-            nm = row["Kernel_Name"]
-            kind = ("spmv" if ("k_spmv_strip" in nm or "k_delta_spmv" in nm) else "fixup" if "k_spmv_fixup" in nm
-                    else "copy" if "k_stream_copy" in nm else None)
+            kind = kernel_kind(row["Kernel_Name"])
             if kind:
                 acc[kind][row["Counter_Name"]].append(float(row["Counter_Value"]))
         return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
@@ -259,8 +267,11 @@ def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
     f, w = per_kernel(fetch_csv), per_kernel(write_csv)
     f_factor = copy_bytes / (f["copy"]["FETCH_SIZE"] * 1024)
     w_factor = copy_bytes / (w["copy"]["WRITE_SIZE"] * 1024)
-    fetch = sum(f[k]["FETCH_SIZE"] for k in ("spmv", "fixup") if k in f) * 1024
-    write = sum(w[k]["WRITE_SIZE"] for k in ("spmv", "fixup") if k in w) * 1024
+    # the launches of one product: SpMV (+ heavy-strip fix-up), SpMM, or the
+    # batched launch (+ its fix-up)
+    kinds = ("spmm",) if "spmm" in f else ("batch", "batch_fixup") if "batch" in f else ("spmv", "fixup")
+    fetch = sum(f[k]["FETCH_SIZE"] for k in kinds if k in f) * 1024
+    write = sum(w[k]["WRITE_SIZE"] for k in kinds if k in w) * 1024
     rec = {"fetch_kib": round(fetch / 1024, 1), "write_kib": round(write / 1024, 1),
            "fetch_factor": round(f_factor, 4), "write_factor": round(w_factor, 4),
            "traffic_bytes": int(fetch * f_factor + write * w_factor), "command": command,
@@ -271,9 +282,9 @@ def summarize_pmc(key, fetch_csv, write_csv, sizes_csv=None, command=""):
         def by_size(d):
             return (32 * d.get("TCC_EA0_RDREQ_32B_sum", 0) + 64 * d.get("TCC_EA0_RDREQ_64B_sum", 0)
                     + 128 * d.get("TCC_EA0_RDREQ_128B_sum", 0))
-        rec["read_bytes_by_request_size"] = int(sum(by_size(z[k]) for k in ("spmv", "fixup") if k in z))
+        rec["read_bytes_by_request_size"] = int(sum(by_size(z[k]) for k in kinds if k in z))
         rec["copy_read_bytes_by_request_size"] = int(by_size(z["copy"]))
-        rec["read_requests"] = {c: int(sum(z[k].get(c, 0) for k in ("spmv", "fixup") if k in z))
+        rec["read_requests"] = {c: int(sum(z[k].get(c, 0) for k in kinds if k in z))
                                 for c in ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum",
                                           "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")}
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -350,7 +361,7 @@ def run_rmat(args, hip):
         cands = [pick]
     else:
         pick, tname = tuned_pick(A, s_allowed, hip)
-        cands = [c for c in tuner.candidates() if c[0] in s_allowed or c == tuner.DELTA]
+        cands = [c for c in tuner.candidates() if c[0] in s_allowed or c[0] == 0]
         if args.s is not None:  # a restricted run: the balanced variant only, plus the pick
             cands = [c for c in cands if c[1] == "balanced" or c == pick]
     res, x = measure_candidates(A, cands, args, hip, stream, check=not args.no_check)
@@ -391,11 +402,13 @@ def run_spmm(args, hip):
     s = args.spmm_s
     d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
     out_k = {}
-    for prec in ("f64", "f32"):
+    precs = ("f64", "f32") if args.spmm_prec is None else (args.spmm_prec,)
+    ks = (1, 4, 8, 16, 32) if args.spmm_k is None else (args.spmm_k,)
+    for prec in precs:
         M = AbhsfMatrix(block_size=s, precision=prec, stream=stream)
         M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
         st = M.stats()
-        for k in (1, 4, 8, 16, 32):
+        for k in ks:
             X = np.random.default_rng(k).uniform(-1, 1, (A.n, k))
             dX, dY = hip.DeviceArray.from_numpy(X), hip.DeviceArray.zeros((A.m, k))
             for _ in range(args.warmup):
@@ -419,14 +432,16 @@ def run_spmm(args, hip):
                 err = max(err, float(np.max(np.abs(Y[:, j] - ref)) / np.max(np.abs(ref))))
             algo = st["stored_bytes"] + 8 * A.n * k + 16 * A.m * k
             out_k[f"{prec}/k{k}"] = {"gflops": round(2 * A.nnz * k / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 4),
-                                     "gbs": round(algo / (ms * 1e-3) / 1e9, 1), "maxnorm_rel_err": err}
+                                     "gbs": round(algo / (ms * 1e-3) / 1e9, 1), "maxnorm_rel_err": err,
+                                     "algo_bytes": algo}
             log(f"{prec} k={k}: {out_k[f'{prec}/k{k}']}")
         M.release()
     best = max(out_k.values(), key=lambda r: r["gflops"])
     out = base_record(args, best["gflops"], best["ms"])
-    out["config"] = {"workload": f"27-point stencil 160^3 (nnz {A.nnz}), k in 1,4,8,16,32, s={s}",
+    out["config"] = {"workload": f"27-point stencil 160^3 (nnz {A.nnz}), k in {','.join(map(str, ks))}, s={s}",
                      "nnz": A.nnz}
-    out["roofline"] = roofline(out_k["f64/k32"])
+    kmax = f"{precs[0]}/k{ks[-1]}"
+    out["roofline"] = roofline(out_k[kmax], f"spmm/{s}/{kmax}")
     out["per_k"] = out_k
     return out
 
@@ -514,10 +529,13 @@ def main():
     ap.add_argument("--workload", default="rmat", choices=["rmat", "block_mix", "spmm", "batch"])
     ap.add_argument("--s", default=None, help="block sizes to measure (block_mix default 16,32,64,128; "
                     "rmat default all tuner candidates)")
-    ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel", "tiled", "delta"],
+    ap.add_argument("--variant", default=None, choices=["strip", "balanced", "panel", "tiled", "delta",
+                                                         "delta_u32_row", "delta_u32_block"],
                     help="block_mix / rmat: measure only (first --s, VARIANT), no tuner (profiler passes)")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--spmm-s", type=int, default=64)
+    ap.add_argument("--spmm-k", type=int, default=None, help="spmm: one right-hand-side count (profiler passes)")
+    ap.add_argument("--spmm-prec", default=None, choices=["f64", "f32"], help="spmm: one value precision")
     ap.add_argument("--batch-count", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")

@@ -225,6 +225,11 @@ abhsf_status abhsf_batch_destroy(abhsf_batch *b);
 int64_t abhsf_mlp_num_params(int32_t nfeat);
 abhsf_status abhsf_mlp_forward(const float *theta, int32_t nfeat, const float *X, int64_t nsamples, float *H1,
                                float *H2, float *out, void *stream);
+/* Ensemble inference in one launch: out (FP64, nsamples) = mean over the
+ * nmembers parameter vectors thetas[m * num_params ..] of the forward
+ * outputs, summed in member order. */
+abhsf_status abhsf_mlp_predict(const float *thetas, int32_t nmembers, int32_t nfeat, const float *X,
+                               int64_t nsamples, double *out, void *stream);
 /* grad (num_params) = dLoss/dtheta, summed over samples in sample order. */
 abhsf_status abhsf_mlp_backward(const float *theta, int32_t nfeat, const float *X, int64_t nsamples,
                                 const float *H1, const float *H2, const float *out, const float *y,
@@ -260,16 +265,37 @@ abhsf_status abhsf_mlp_train_noalloc(float *theta, int32_t nfeat, const float *X
  * delta_matrix::from_elements / spmv / release (:635-710, 712-789, 790-800). */
 typedef struct abhsf_delta abhsf_delta;
 
+enum {
+    ABHSF_DELTA_VARINT = 0,      /* tagged varint words of key = (row << K) + col (:859-944) */
+    ABHSF_DELTA_U32_ROW = 1,     /* u32 deltas of key = (row << 31) + col, row-major order
+                                    (spmv/orig/delta.cpp:124-151) */
+    ABHSF_DELTA_U32_BLOCK = 2    /* u32 deltas of the 8 x 8 cache-block key ((row/8) << 31) +
+                                    ((col/8) << 6) + ((row%8) << 3) + col%8, ordered by
+                                    (row/8, col/8, row%8, col%8) (:636-689); n <= 2^28 */
+};
+
 typedef struct abhsf_delta_stats {
     int64_t m, n, nnz;
-    int32_t key_bits;          /* K of key = (row << K) + col */
+    int32_t key_bits;          /* varint: K of key = (row << K) + col; u32 codecs: 31 */
     int32_t chunk;             /* elements per chunk */
-    uint64_t words[4];         /* elements coded in 1 / 2 / 4 / 8 bytes */
+    uint64_t words[4];         /* varint: elements coded in 1 / 2 / 4 / 8 bytes; u32 codecs:
+                                  words[2] elements with a u32 delta, words[3] exceptions */
     uint64_t stored_bytes;     /* all arrays as stored in HBM */
     double build_ms;           /* host encode + upload */
+    int32_t codec;             /* ABHSF_DELTA_* */
+    int32_t reserved;
 } abhsf_delta_stats;
 
+/* A varint handle (ABHSF_DELTA_VARINT). */
 abhsf_status abhsf_delta_create(void *stream, abhsf_delta **out);
+/* A handle for one of the ABHSF_DELTA_* codecs.  The u32 codecs keep the
+ * reference's fixed-width words (replacing delta_matrix of
+ * spmv/orig/delta.cpp:118-160 and the cache-blocking delta_matrix of
+ * spmv/unified/spmv.cpp:628-710); where the reference asserts that a delta
+ * fits 32 bits, a chunk's first element moves its key into the chunk base
+ * (delta 0) and any other element stores 0 -- never a real delta there --
+ * with its key in an exception list. */
+abhsf_status abhsf_delta_create_codec(void *stream, int32_t codec, abhsf_delta **out);
 /* CSR input as for abhsf_build_csr; the encoder runs on the host. */
 abhsf_status abhsf_delta_build_csr(abhsf_delta *d, int64_t m, int64_t n, int64_t nnz, const int64_t *rowptr,
                                    const int32_t *col, const double *val, int32_t on_device);
@@ -277,8 +303,10 @@ abhsf_status abhsf_delta_build_csr(abhsf_delta *d, int64_t m, int64_t n, int64_t
  * accumulating delta_matrix::spmv(x, y). */
 abhsf_status abhsf_delta_spmv(abhsf_delta *d, double alpha, const double *x, double beta, double *y);
 abhsf_status abhsf_delta_get_stats(const abhsf_delta *d, abhsf_delta_stats *out);
-/* Arrays: "tags" u32, "w8" u8, "w16" u16, "w32" u32, "w64" u64, "base" u64,
- * "start" u32[chunks][4], "val" f64.  dst NULL: size only. */
+/* Arrays: varint "tags" u32, "w8" u8, "w16" u16, "w32" u32, "w64" u64,
+ * "start" u32[chunks][4]; u32 codecs "da" u32[nnz], "xstart" u32[chunks + 1],
+ * "exc" u64; all "base" u64[chunks], "val" f64 (element order).  dst NULL:
+ * size only. */
 abhsf_status abhsf_delta_export(const abhsf_delta *d, const char *name, void *dst, size_t cap, size_t *bytes);
 abhsf_status abhsf_delta_synchronize(abhsf_delta *d);
 abhsf_status abhsf_delta_destroy(abhsf_delta *d);

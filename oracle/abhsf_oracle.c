@@ -25,6 +25,11 @@
  *                      sums, 88.cpp y-sum); brute-force dense product on tiny
  *                      matrices.
  *   orc_rows_stdev     App. B "0.763763" for test.mtx; two-pass formula.
+ *   orc_varint_*       hand-encoded words at every width boundary; round trip.
+ *   orc_delta32_*      keys of hand-placed elements worked out by hand for both
+ *                      layouts (tests/test_oracle.py), the element order of a
+ *                      hand-listed 16 x 16 case, escapes at a known row gap,
+ *                      round trip on seeded matrices.
  */
 #include "abhsf_oracle.h"
 
@@ -640,4 +645,76 @@ uint64_t orc_varint_decode(uint64_t nnz, uint64_t first, const uint8_t *in, uint
         pos += (uint64_t)bytes;
     }
     return pos;
+}
+
+
+/* ---- u32 delta codecs (spmv/orig/delta.cpp:124-151; spmv/unified/spmv.cpp:636-689) ---- */
+
+static uint64_t d32_key(int layout, uint64_t row, uint64_t col)
+{
+    if (layout == 1) return (row << 31) + col;
+    return ((row >> 3) << 31) + ((col >> 3) << 6) + ((row & 7) << 3) + (col & 7);
+}
+
+static void d32_rowcol(int layout, uint64_t key, int64_t *row, int64_t *col)
+{
+    if (layout == 1) {
+        *row = (int64_t)(key >> 31);
+        *col = (int64_t)(key & ((1ull << 31) - 1));
+    } else {
+        *row = (int64_t)(((key >> 31) << 3) | ((key >> 3) & 7));
+        *col = (int64_t)((((key >> 6) & ((1ull << 25) - 1)) << 3) | (key & 7));
+    }
+}
+
+uint64_t orc_delta32_encode(int layout, int64_t m, const int64_t *rowptr, const int32_t *col, uint64_t chunk,
+                            uint32_t *da, uint64_t *perm, uint64_t *base, uint32_t *xstart, uint64_t *exc)
+{
+    const uint64_t nnz = (uint64_t)rowptr[m];
+    uint64_t *key = (uint64_t *)malloc((nnz ? nnz : 1) * sizeof(uint64_t));
+    uint64_t *pay = (uint64_t *)malloc((nnz ? nnz : 1) * sizeof(uint64_t));
+    for (int64_t r = 0; r < m; r++)
+        for (int64_t e = rowptr[r]; e < rowptr[r + 1]; e++) {
+            key[e] = d32_key(layout, (uint64_t)r, (uint64_t)col[e]);
+            pay[e] = (uint64_t)e;
+        }
+    /* layout 1 keys are already ascending in CSR order; layout 2 is sorted by
+       key, which orders (row/8, col/8, row%8, col%8) lexicographically */
+    if (layout != 1) radix_sort_u64(key, pay, nnz, 64);
+    uint64_t prev = 0, nx = 0;
+    for (uint64_t e = 0; e < nnz; e++) {
+        const uint64_t d = key[e] - prev;
+        if (e % chunk == 0) {
+            /* a chunk's first element never escapes: its base absorbs a
+               delta too wide for 32 bits */
+            base[e / chunk] = d < (1ull << 32) ? prev : key[e];
+            da[e] = d < (1ull << 32) ? (uint32_t)d : 0u;
+            xstart[e / chunk] = (uint32_t)nx;
+        } else if (d >= (1ull << 32)) {
+            da[e] = 0u;
+            exc[nx++] = key[e];
+        } else {
+            da[e] = (uint32_t)d;
+        }
+        perm[e] = pay[e];
+        prev = key[e];
+    }
+    xstart[(nnz + chunk - 1) / chunk] = (uint32_t)nx;
+    free(key);
+    free(pay);
+    return nx;
+}
+
+void orc_delta32_decode(int layout, uint64_t nnz, uint64_t chunk, const uint32_t *da, const uint64_t *base,
+                        const uint32_t *xstart, const uint64_t *exc, int64_t *rows, int64_t *cols)
+{
+    uint64_t key = 0, x = 0;
+    for (uint64_t e = 0; e < nnz; e++) {
+        if (e % chunk == 0) {
+            key = base[e / chunk];
+            x = xstart[e / chunk];
+        }
+        key = (da[e] || e % chunk == 0) ? key + da[e] : exc[x++];
+        d32_rowcol(layout, key, &rows[e], &cols[e]);
+    }
 }

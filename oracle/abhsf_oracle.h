@@ -124,6 +124,30 @@ uint64_t orc_varint_encode(uint64_t nnz, const uint64_t *keys, uint8_t *out);
  * consumed. */
 uint64_t orc_varint_decode(uint64_t nnz, uint64_t first, const uint8_t *in, uint64_t *keys);
 
+/* Fixed-width u32 delta codecs of linearised keys.
+ * layout 1 (spmv/orig/delta.cpp:124-151): row-major order,
+ *   key = (row << 31) + col;
+ * layout 2 (spmv/unified/spmv.cpp:636-689, 8x8 cache blocks): order by
+ *   (row / 8, col / 8, row % 8, col % 8), key = ((row / 8) << 31) +
+ *   ((col / 8) << 6) + ((row % 8) << 3) + col % 8 (needs n <= 2^28).
+ * Element e (in key order) stores da[e] = key_e - key_{e-1} as a u32, with
+ * key_{-1} = 0.  Defined by this build (the reference asserts instead):
+ * elements come in chunks of `chunk`; base[c] is the key before chunk c's
+ * first element, or that element's own key (and its da 0) when its delta
+ * needs more than 32 bits.  Any other element whose delta needs more than 32
+ * bits stores 0 -- never a real delta there, keys strictly increase -- and
+ * its key goes to the exception list exc, in element order.  xstart[c] is the
+ * chunk's first exception (xstart[nchunk] =
+ * exceptions in all).  perm[e] = CSR position of element e.  Inputs: CSR
+ * with columns ascending within rows.  Arrays sized nnz / nnz / nchunk /
+ * nchunk + 1 (exc: nnz at most).  Returns the number of exceptions. */
+uint64_t orc_delta32_encode(int layout, int64_t m, const int64_t *rowptr, const int32_t *col, uint64_t chunk,
+                            uint32_t *da, uint64_t *perm, uint64_t *base, uint32_t *xstart, uint64_t *exc);
+
+/* Running-sum decode of the same stream: rows[e], cols[e] of element e. */
+void orc_delta32_decode(int layout, uint64_t nnz, uint64_t chunk, const uint32_t *da, const uint64_t *base,
+                        const uint32_t *xstart, const uint64_t *exc, int64_t *rows, int64_t *cols);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -72,6 +72,10 @@ def lib():
         L.orc_varint_encode.argtypes = [u64, vp, vp]
         L.orc_varint_decode.restype = u64
         L.orc_varint_decode.argtypes = [u64, u64, vp, vp]
+        L.orc_delta32_encode.restype = u64
+        L.orc_delta32_encode.argtypes = [C.c_int, C.c_int64, vp, vp, u64, vp, vp, vp, vp, vp]
+        L.orc_delta32_decode.restype = None
+        L.orc_delta32_decode.argtypes = [C.c_int, u64, u64, vp, vp, vp, vp, vp, vp]
         L.orc_rows_stdev.restype = d
         L.orc_rows_stdev.argtypes = [u64, vp]
         L.orc_num_threads.restype = C.c_int
@@ -140,6 +144,39 @@ def varint_decode(stream, first, nnz):
     keys = np.zeros(nnz, dtype=np.uint64)
     used = lib().orc_varint_decode(nnz, int(first), _p(stream), _p(keys))
     return keys, int(used)
+
+
+DELTA32_ROWMAJOR, DELTA32_CACHEBLOCK = 1, 2
+DELTA_CHUNK = 512
+
+
+def delta32_encode(layout, m, rowptr, col, chunk=DELTA_CHUNK):
+    """u32 delta stream of the keys of layout 1 (row << 31 | col,
+    spmv/orig/delta.cpp:124-151) or 2 (8x8 cache blocks,
+    spmv/unified/spmv.cpp:636-689); returns dict da, perm, base, xstart, exc."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    nnz = int(rowptr[m])
+    nch = (nnz + chunk - 1) // chunk
+    da = np.zeros(max(nnz, 1), dtype=np.uint32)
+    perm = np.zeros(max(nnz, 1), dtype=np.uint64)
+    base = np.zeros(max(nch, 1), dtype=np.uint64)
+    xstart = np.zeros(nch + 1, dtype=np.uint32)
+    exc = np.zeros(max(nnz, 1), dtype=np.uint64)
+    nx = lib().orc_delta32_encode(layout, m, _p(rowptr), _p(col), chunk, _p(da), _p(perm), _p(base), _p(xstart),
+                                  _p(exc))
+    return {"da": da[:nnz], "perm": perm[:nnz], "base": base[:nch], "xstart": xstart, "exc": exc[:nx]}
+
+
+def delta32_decode(layout, enc, chunk=DELTA_CHUNK):
+    nnz = enc["da"].size
+    rows = np.zeros(max(nnz, 1), dtype=np.int64)
+    cols = np.zeros(max(nnz, 1), dtype=np.int64)
+    arrs = [np.ascontiguousarray(enc[k], dtype=t) for k, t in
+            (("da", np.uint32), ("base", np.uint64), ("xstart", np.uint32), ("exc", np.uint64))]
+    arrs = [a if a.size else np.zeros(1, dtype=a.dtype) for a in arrs]
+    lib().orc_delta32_decode(layout, nnz, chunk, *(_p(a) for a in arrs), _p(rows), _p(cols))
+    return rows[:nnz], cols[:nnz]
 
 
 def csr_spmv(rowptr, col, val, x, alpha=1.0, beta=0.0, y=None):

@@ -106,3 +106,54 @@ def test_delta_from_elements_accumulates(gpu):
         D.spmv(x, y)
     ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, np.full(A.n, 0.5))
     assert abs(np.linalg.norm(y.numpy()) / 25 - np.linalg.norm(ref)) <= 1e-12 * np.linalg.norm(ref)
+
+
+U32_MATS = {
+    "rmat_14": lambda: synth.rmat(14, 16, seed=2),             # empty rows: escapes in row-major order
+    "laplacian_120": lambda: synth.laplacian2d(120),
+    "block_mix_2^13": lambda: synth.block_mix(1 << 13, 4),
+    "wide_30x200000": lambda: synth.random_sparse(30, 200000, 0.001, seed=7),
+    "sparse_rows": lambda: synth.from_triplets(5000, 700, [0, 0, 1999, 4999], [0, 699, 5, 3], [1.0, 2.0, 3.0, 4.0]),
+    "empty": lambda: synth.from_triplets(64, 64, [], [], []),
+}
+
+
+@pytest.mark.parametrize("codec", ["u32_row", "u32_block"])
+@pytest.mark.parametrize("name", list(U32_MATS))
+def test_u32_codecs_match_oracle_encoder(gpu, codec, name):
+    """The u32 delta codecs (spmv/orig/delta.cpp:124-151 row << 31 | col;
+    spmv/unified/spmv.cpp:636-689 8 x 8 cache blocks): every exported array
+    equals the oracle encoder's, values in the codec's element order, and the
+    SpMV matches the CSR product."""
+    A = U32_MATS[name]()
+    layout = pyoracle.DELTA32_ROWMAJOR if codec == "u32_row" else pyoracle.DELTA32_CACHEBLOCK
+    print(f"case delta {codec} {name}", flush=True)
+    D = DeltaMatrix(codec=codec)
+    D.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    st = D.stats()
+    assert st["codec"] == codec and st["key_bits"] == 31
+    enc = pyoracle.delta32_encode(layout, A.m, A.rowptr, A.col, CHUNK)
+    assert np.array_equal(D.export("da"), enc["da"])
+    assert np.array_equal(D.export("base"), enc["base"])
+    if A.nnz:
+        assert np.array_equal(D.export("xstart"), enc["xstart"])
+    assert np.array_equal(D.export("exc"), enc["exc"])
+    assert np.array_equal(D.export("val"), A.val[enc["perm"].astype(np.int64)])
+    assert st["words"][8] == enc["exc"].size and st["words"][4] == A.nnz - enc["exc"].size
+    x = np.random.default_rng(3).uniform(-1, 1, A.n)
+    y0 = np.random.default_rng(4).uniform(-1, 1, A.m)
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0), (1.0, 1.0)):
+        dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y0)
+        D.spmv_async(alpha, dx, beta, dy)
+        D.synchronize()
+        ref = pyoracle.csr_spmv(A.rowptr, A.col, A.val, x, alpha, beta, y0)
+        assert relerr(dy.numpy(), ref) <= TOL, (alpha, beta)
+    D.release()
+
+
+def test_u32_block_rejects_wide_matrices(gpu):
+    """The cache-block key holds column blocks in 25 bits (n <= 2^28)."""
+    from abhsf_amd import AbhsfError
+    D = DeltaMatrix(codec="u32_block")
+    with pytest.raises(AbhsfError, match="2\\^28"):
+        D.from_csr(1, (1 << 28) + 1, np.array([0, 1]), np.array([5]), np.array([1.0]))

@@ -157,7 +157,7 @@ def test_spmm(gpu, s, nrhs):
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("s", [16, 64, 256])
-@pytest.mark.parametrize("nrhs", [1, 3, 8, 13])
+@pytest.mark.parametrize("nrhs", [1, 3, 8, 13, 16, 20, 32])
 def test_spmm_csr_blocks(gpu, precision, s, nrhs):
     """27-point stencil on 24^3: CSR blocks (s = 64, 256) whose rows each lane
     walks two at a time, plus bitmap blocks (s = 16); 13 right-hand sides

@@ -18,6 +18,12 @@ transposed operand fails at least one):
   varint        hand-encoded words at every width boundary of
                 spmv/unified/spmv.cpp:859-896 (2^6, 2^14, 2^30), the byte
                 order, decode(encode(k)) = k on random and 62-bit keys.
+  delta32       keys and deltas of hand-placed elements worked out by hand for
+                spmv/orig/delta.cpp:124-151 (row << 31 | col) and the 8 x 8
+                cache-block key of spmv/unified/spmv.cpp:636-689 (including
+                an element the block order moves), escapes at known row gaps
+                and at a chunk's first element, the element order checked
+                against a lexicographic sort, round trip on seeded matrices.
 """
 import json
 import os
@@ -346,3 +352,74 @@ def test_varint_codec_roundtrip():
         s = O.varint_encode(keys)
         got, used = O.varint_decode(s, keys[0], keys.size)
         assert used == s.size and np.array_equal(got, keys)
+
+
+def _csr(m, entries):
+    """CSR of (row, col) entries given in row-major order."""
+    rows = [r for r, _ in entries]
+    rowptr = np.zeros(m + 1, dtype=np.int64)
+    for r in rows:
+        rowptr[r + 1] += 1
+    return np.cumsum(rowptr), np.array([c for _, c in entries], dtype=np.int32)
+
+
+def test_delta32_row_major_by_hand():
+    """spmv/orig/delta.cpp:124-151: key = (row << 31) + col, deltas from 0.
+    Keys by hand: (0,1) 1, (0,5) 5, (1,0) 2^31 = 2147483648, (3,2)
+    3*2^31 + 2 = 6442450946, (9,17) 19327352849, (19,19) 40802189331; the
+    last three deltas are 2^32 or more (empty rows skipped), so they escape."""
+    rp, col = _csr(20, [(0, 1), (0, 5), (1, 0), (3, 2), (9, 17), (19, 19)])
+    e = O.delta32_encode(O.DELTA32_ROWMAJOR, 20, rp, col)
+    assert e["da"].tolist() == [1, 4, 2147483643, 0, 0, 0]
+    assert e["exc"].tolist() == [6442450946, 19327352849, 40802189331]
+    assert e["base"].tolist() == [0] and e["xstart"].tolist() == [0, 3]
+    assert e["perm"].tolist() == [0, 1, 2, 3, 4, 5]
+    r, c = O.delta32_decode(O.DELTA32_ROWMAJOR, e)
+    assert r.tolist() == [0, 0, 1, 3, 9, 19] and c.tolist() == [1, 5, 0, 2, 17, 19]
+
+
+def test_delta32_cache_block_by_hand():
+    """spmv/unified/spmv.cpp:636-689: elements ordered by (row/8, col/8,
+    row%8, col%8), key ((row/8) << 31) + ((col/8) << 6) + ((row%8) << 3) +
+    col%8.  By hand: (0,1) 1, (1,0) 8, (7,7) 63, (0,9) 64 + 1 = 65 (block
+    column 1), (8,0) 2^31; so (0,9) moves behind (1,0) and (7,7)."""
+    rp, col = _csr(9, [(0, 1), (0, 9), (1, 0), (7, 7), (8, 0)])
+    e = O.delta32_encode(O.DELTA32_CACHEBLOCK, 9, rp, col)
+    assert e["perm"].tolist() == [0, 2, 3, 1, 4]
+    assert e["da"].tolist() == [1, 7, 55, 2, 2147483583]
+    assert e["exc"].size == 0
+    r, c = O.delta32_decode(O.DELTA32_CACHEBLOCK, e)
+    assert r.tolist() == [0, 1, 7, 0, 8] and c.tolist() == [1, 0, 7, 9, 0]
+    # (9,17) -> block row 1, block column 2, local (1,1): 2^31 + 128 + 8 + 1
+    rp, col = _csr(10, [(9, 17)])
+    assert O.delta32_encode(O.DELTA32_CACHEBLOCK, 10, rp, col)["da"].tolist() == [2147483785]
+
+
+def test_delta32_chunk_first_escape_by_hand():
+    """Chunks of 2: a chunk's first element whose delta needs 33+ bits puts
+    its key into the chunk base (delta 0); any other element escapes."""
+    rp, col = _csr(10, [(0, 0), (0, 1), (5, 0), (9, 3)])
+    e = O.delta32_encode(O.DELTA32_ROWMAJOR, 10, rp, col, chunk=2)
+    assert e["da"].tolist() == [0, 1, 0, 0]
+    assert e["base"].tolist() == [0, 5 * 2 ** 31]
+    assert e["xstart"].tolist() == [0, 0, 1]
+    assert e["exc"].tolist() == [9 * 2 ** 31 + 3]
+    r, c = O.delta32_decode(O.DELTA32_ROWMAJOR, e, chunk=2)
+    assert r.tolist() == [0, 0, 5, 9] and c.tolist() == [0, 1, 0, 3]
+
+
+@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("chunk", [512, 7])
+def test_delta32_roundtrip_and_order(layout, chunk):
+    from abhsf_amd import synth
+    for A in (synth.rmat(12, 8, seed=1), synth.laplacian2d(40), synth.random_sparse(300, 5000, 0.01, seed=2)):
+        e = O.delta32_encode(layout, A.m, A.rowptr, A.col, chunk)
+        r, c = O.delta32_decode(layout, e, chunk)
+        rr = np.repeat(np.arange(A.m), np.diff(A.rowptr))
+        p = e["perm"].astype(np.int64)
+        assert np.array_equal(r, rr[p]) and np.array_equal(c, A.col[p].astype(np.int64))
+        assert np.array_equal(np.sort(p), np.arange(A.nnz))  # a permutation
+        # order: row-major, or (row/8, col/8, row%8, col%8) lexicographic
+        key = (r, c) if layout == 1 else (r // 8, c // 8, r % 8, c % 8)
+        order = np.lexsort(key[::-1])
+        assert np.array_equal(order, np.arange(A.nnz))

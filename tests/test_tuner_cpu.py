@@ -74,7 +74,7 @@ def test_feature_vector_length_matches_nfeat():
     sf = {"log2_s": 3.0, "log2_blk_per_entry": 0.0, "blk_frac": [1, 0, 0, 0], "ent_frac": [1, 0, 0, 0],
           "log2_bytes_per_entry": 3.5, "strip_cv": 0.0, "log2_max_strip": 0.0, "heavy_frac": 0.0, "log2_M": 4.0,
           "tile_frac": 0.5}
-    for v in tuner.VARIANTS + ("delta",):
+    for v in tuner.VARIANTS + tuple(tuner.DELTA_CODECS):
         assert tuner.feature_vector(g, sf, v).shape == (tuner.NFEAT,)
 
 
@@ -111,3 +111,22 @@ def test_tile_fraction_brute_force(seed):
                 if w <= 2 * tuner.unit_entries(s) and all(tuner.choose(s, z) == 0 for z in zs.values()):
                     pure += w
             assert got == pytest.approx(pure / tot if tot else 0.0), (s, stride)
+
+
+@pytest.mark.parametrize("name", ["rmat_13", "laplacian_60", "sparse_rows", "band_gaps", "one"])
+def test_u32_delta_features_match_oracle_encoder(name):
+    """The tuner's closed-form counts for the u32 delta codecs (escapes, stored
+    bytes) equal what the oracle's encoder produces."""
+    from oracle import pyoracle
+    A = {"rmat_13": lambda: synth.rmat(13, 8, seed=5), "laplacian_60": lambda: synth.laplacian2d(60),
+         "sparse_rows": lambda: synth.from_triplets(5000, 700, [0, 0, 1999, 4999], [0, 699, 5, 3], [1.0] * 4),
+         "band_gaps": lambda: synth.from_triplets(100, 100, [0, 7, 16, 17, 40, 41, 99], [50, 1, 60, 2, 0, 99, 99],
+                                                  [1.0] * 7),
+         "one": lambda: synth.from_triplets(3, 3, [2], [2], [1.0])}[name]()
+    for codec, layout in (("u32_row", pyoracle.DELTA32_ROWMAJOR), ("u32_block", pyoracle.DELTA32_CACHEBLOCK)):
+        f = tuner.delta_features(A.rowptr, A.col, A.n, codec)
+        enc = pyoracle.delta32_encode(layout, A.m, A.rowptr, A.col)
+        nexc = enc["exc"].size
+        assert round(f["ent_frac"][3] * A.nnz) == nexc, codec
+        chunks = -(-A.nnz // 512)
+        assert f["stored_bytes"] == 12 * A.nnz + 8 * nexc + 8 * chunks + 4 * (chunks + 1)

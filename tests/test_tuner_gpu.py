@@ -173,3 +173,26 @@ def test_blockstats_all_64_pairs_and_tables(gpu):
         t = blockstats.mmf_table(A, hg)
         ho = {k: pyoracle.block_hist(r, c, k[0].bit_length() - 1, k[1].bit_length() - 1) for k in hg}
         assert t == {k: pyoracle.mmf_row(A.m, A.n, A.nnz, k[0], k[1], ho[k]) for k in hg}
+
+
+def test_fused_ensemble_predict_matches_member_forwards(gpu):
+    """abhsf_mlp_predict (every member and sample in one launch) equals the
+    mean of the members' abhsf_mlp_forward outputs: the same float
+    operations in the same order, summed over members in FP64."""
+    rng = np.random.default_rng(12)
+    F, B, nm = tuner.NFEAT, 27, 5
+    thetas = np.stack([tuner.init_theta(F, seed) for seed in range(nm)]).astype(np.float32)
+    X = rng.standard_normal((B, F)).astype(np.float32)
+    dx = DeviceArray.from_numpy(X)
+    H1, H2, out = DeviceArray((B, 64), np.float32), DeviceArray((B, 64), np.float32), DeviceArray((B,), np.float32)
+    acc = np.zeros(B)
+    for th in thetas:
+        dt = DeviceArray.from_numpy(np.ascontiguousarray(th))
+        capi.call("abhsf_mlp_forward", dt.ptr, F, dx.ptr, B, H1.ptr, H2.ptr, out.ptr, None)
+        hip.synchronize()
+        acc += out.numpy().astype(np.float64)
+    dts = DeviceArray.from_numpy(thetas)
+    dout = DeviceArray((B,), np.float64)
+    capi.call("abhsf_mlp_predict", dts.ptr, nm, F, dx.ptr, B, dout.ptr, None)
+    hip.synchronize()
+    assert np.allclose(dout.numpy(), acc / nm, rtol=1e-12, atol=0.0)
