@@ -34,6 +34,7 @@ struct abhsf_handle {
     abhsf_options opts;
     hipStream_t stream;
     abhsf::Storage S;
+    abhsf::DevBuf col_x, col_y;  // tiled SpMM: one right-hand side at a time, made contiguous
     int32_t symmetry = ABHSF_GENERAL;  // applies to the next build
     int32_t built_symmetry = ABHSF_GENERAL;
 };
@@ -48,8 +49,9 @@ struct abhsf_batch {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<abhsf::BatchEntry> host;
+    std::vector<const abhsf_handle *> mats;  // members (their storage must outlive the batch)
     abhsf::DevBuf table, heavy, item_mat;
-    int64_t total_items = 0, nheavy = 0;
+    int64_t total_items = 0, items_f64 = 0, nheavy = 0;
     bool vectors_set = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -301,8 +303,22 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
         ABHSF_REQUIRE(h->S.built, ABHSF_ERR_STATE, "matrix not built");
         ABHSF_REQUIRE(nrhs >= 1 && nrhs <= 64, ABHSF_ERR_UNSUPPORTED, "nrhs must be in [1, 64]");
         ABHSF_REQUIRE(ldx >= nrhs && ldy >= nrhs, ABHSF_ERR_INVALID, "leading dimension smaller than nrhs");
-        ABHSF_REQUIRE(!h->S.tiled || (nrhs == 1 && ldx == 1 && ldy == 1), ABHSF_ERR_UNSUPPORTED,
-                      "a tiled matrix supports SpMV only (one contiguous right-hand side)");
+        if (h->S.tiled && !(nrhs == 1 && ldx == 1 && ldy == 1)) {
+            // tiles reorder the COO blocks, which the SpMM kernels walk strip by
+            // strip: each right-hand side goes through the tiled SpMV kernels,
+            // gathered into and scattered from contiguous vectors the handle
+            // keeps (allocated at the first such call)
+            if (h->col_x.bytes() < (size_t)h->S.n * 8) h->col_x.alloc((size_t)h->S.n * 8);
+            if (h->col_y.bytes() < (size_t)h->S.m * 8) h->col_y.alloc((size_t)h->S.m * 8);
+            double *cx = h->col_x.get<double>(), *cy = h->col_y.get<double>();
+            for (int32_t j = 0; j < nrhs; j++) {
+                abhsf::strided_copy(X + j, ldx, cx, 1, h->S.n, h->stream);
+                if (beta != 0.0) abhsf::strided_copy(Y + j, ldy, cy, 1, h->S.m, h->stream);
+                abhsf::spmm(h->S.launch(), h->S.view(), 1, alpha, cx, 1, beta, cy, 1, h->stream);
+                abhsf::strided_copy(cy, 1, Y + j, ldy, h->S.m, h->stream);
+            }
+            return;
+        }
         abhsf::spmm(h->S.launch(), h->S.view(), nrhs, alpha, X, ldx, beta, Y, ldy, h->stream);
         if (h->built_symmetry != ABHSF_GENERAL && alpha != 0.0)  // the mirrored triangle
             abhsf::spmv_transposed(h->S.launch(), h->S.view(), nrhs,
@@ -427,6 +443,22 @@ abhsf_status abhsf_mlp_train_noalloc(float *theta, int32_t nfeat, const float *X
     return ABHSF_OK;
 }
 
+namespace {
+// Symmetric members: the mirrored triangle's pass (y_i += +-alpha * T_off^T
+// x_i, global atomics) after the batched launch, one launch per such member.
+void batch_mirrored(abhsf_batch *b, double alpha)
+{
+    if (alpha == 0.0) return;
+    for (size_t i = 0; i < b->mats.size(); i++) {
+        const abhsf_handle *h = b->mats[i];
+        if (h->built_symmetry == ABHSF_GENERAL) continue;
+        abhsf::spmv_transposed(h->S.launch(), h->S.view(), 1,
+                               h->built_symmetry == ABHSF_SKEW_SYMMETRIC ? -alpha : alpha, b->host[i].x, 1,
+                               b->host[i].y, 1, b->stream);
+    }
+}
+}  // namespace
+
 abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *stream, abhsf_batch **out)
 {
     return guarded([&] {
@@ -443,30 +475,34 @@ abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *
         for (int32_t i = 0; i < count; i++) {
             const abhsf_handle *h = mats[i];
             ABHSF_REQUIRE(h && h->S.built, ABHSF_ERR_STATE, "batch member not built");
-            ABHSF_REQUIRE(h->S.val_bytes == 8, ABHSF_ERR_UNSUPPORTED, "batched launch needs FP64 storage");
-            ABHSF_REQUIRE(h->built_symmetry == ABHSF_GENERAL, ABHSF_ERR_UNSUPPORTED,
-                          "batched launch takes general (both-triangle) storage");
             abhsf::BatchEntry e{};
             e.A = h->S.view();
             e.B = h->S.launch().bal;
             if (!e.B.light && e.B.n_units == 0) e.B.n_light = h->S.M;
             e.code = abhsf::batch_code(h->S.bcol_bytes, h->S.s);
+            e.comp = h->S.val_bytes == 4 ? 1 : 0;
             for (int64_t q = 0; q < e.B.n_heavy; q++) hv.insert(hv.end(), {(uint32_t)i, (uint32_t)q});
             b->host.push_back(e);
+            b->mats.push_back(h);
         }
-        // number the items member by member, longest first: members in
-        // decreasing order of entries per work item (one kernel serves all
-        // index-type combinations; the grid-stride loop starts the items in
-        // this order, so the long ones do not form the launch's tail)
+        // number the items member by member, FP64 members first (one launch),
+        // then FP32 members (a second launch); within each, longest first:
+        // members in decreasing order of entries per work item (one kernel
+        // serves all index-type combinations; the grid-stride loop starts the
+        // items in this order, so the long ones do not form the launch's tail)
         std::vector<size_t> ord(b->host.size());
         std::iota(ord.begin(), ord.end(), (size_t)0);
         auto per_item = [&](size_t i) {
             const auto &e = b->host[i];
             return (double)mats[i]->S.nnz / (double)std::max<int64_t>(1, e.B.n_light + e.B.n_units);
         };
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t c) { return per_item(a) > per_item(c); });
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t c) {
+            if (b->host[a].comp != b->host[c].comp) return b->host[a].comp < b->host[c].comp;
+            return per_item(a) > per_item(c);
+        });
         int64_t item = 0;
         for (size_t i : ord) {
+            if (!b->host[i].comp) b->items_f64 = item + b->host[i].B.n_light + b->host[i].B.n_units;
             b->host[i].item0 = item;
             item += b->host[i].B.n_light + b->host[i].B.n_units;
         }
@@ -511,8 +547,9 @@ abhsf_status abhsf_batch_spmv(abhsf_batch *b, double alpha, double beta, int32_t
         ABHSF_REQUIRE(b, ABHSF_ERR_INVALID, "null batch");
         ABHSF_REQUIRE(b->vectors_set, ABHSF_ERR_STATE, "abhsf_batch_set_vectors not called");
         auto launch = [&] {
-            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
-                              b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream);
+            abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->items_f64,
+                              b->total_items, b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream);
+            batch_mirrored(b, alpha);
         };
         if (!use_graph) {
             launch();
@@ -557,9 +594,10 @@ abhsf_status abhsf_batch_profile(abhsf_batch *b, double alpha, double beta, uint
         const size_t nm = b->host.size();
         abhsf::DevBuf cyc(nm * sizeof(unsigned long long));
         ABHSF_HIP(hipMemsetAsync(cyc.get(), 0, cyc.bytes(), b->stream));
-        abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->total_items,
-                          b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream,
+        abhsf::batch_spmv(b->table.get<abhsf::BatchEntry>(), b->item_mat.get<uint32_t>(), b->items_f64,
+                          b->total_items, b->heavy.get<uint32_t>(), b->nheavy, alpha, beta, b->stream,
                           cyc.get<unsigned long long>());
+        batch_mirrored(b, alpha);  // results complete; its time is not in the members' cycles
         ABHSF_HIP(hipMemcpyAsync(cycles, cyc.get(), cyc.bytes(), hipMemcpyDeviceToHost, b->stream));
         ABHSF_HIP(hipStreamSynchronize(b->stream));
     });

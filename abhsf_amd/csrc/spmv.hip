@@ -853,6 +853,7 @@ __device__ inline BatchEntry load_entry(const BatchEntry *p)
     e.y = ld_const(&p->y);
     e.item0 = ld_const(&p->item0);
     e.code = ld_const(&p->code);
+    e.comp = ld_const(&p->comp);
     return e;
 }
 
@@ -862,12 +863,14 @@ __device__ inline BatchEntry load_entry(const BatchEntry *p)
 // decode (a uniform branch), so one launch serves every member.  FP64
 // values.  TIMED: each wave also adds the shader-clock cycles of its item to
 // cyc[matrix] (the per-matrix cost inside a full launch).
-template <bool TIMED>
+template <bool TIMED, bool COMP>
 __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict__ E,
-                                                    const uint32_t *__restrict__ item_mat, int64_t G, double alpha,
-                                                    double beta, unsigned long long *cyc)
+                                                    const uint32_t *__restrict__ item_mat, int64_t g0, int64_t G,
+                                                    double alpha, double beta, unsigned long long *cyc)
 {
+    using VT = typename std::conditional<COMP, float, double>::type;
     constexpr int kWaves = 4;
+    static_assert(kTileRows >= 2 * kTileRowsComp, "compensated accumulators: hi and lo halves");
     __shared__ double s_acc[kWaves][kTileRows];
     __shared__ uint32_t s_end[kWaves][kWave];
     __shared__ uint32_t s_bc[kWaves][kWave];
@@ -876,7 +879,7 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)gridDim.x * kWaves;
 #pragma unroll 1
-    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < G; g += nw) {
+    for (int64_t g = g0 + (int64_t)blockIdx.x * kWaves + wave; g < G; g += nw) {
         const int mat = __builtin_amdgcn_readfirstlane((int)item_mat[g]);
         // the table is read-only during the launch: its fields are read through
         // the constant address space (scalar loads into SGPRs, as the
@@ -889,27 +892,27 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
         uint32_t *se = s_end[wave], *sb = s_bc[wave], *ss = s_st[wave], *so = s_to[wave];
         switch (e.code) {  // batch_code: (bcol u32 ? 3 : 0) + (s <= 8 ? 0 : s <= 16 ? 1 : 2)
         case 0:
-            spmv_item<double, uint16_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+            spmv_item<VT, uint16_t, uint8_t, uint8_t, COMP>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
                                                                  ss, so, lane);
             break;
         case 1:
-            spmv_item<double, uint16_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+            spmv_item<VT, uint16_t, uint8_t, uint16_t, COMP>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
                                                                   ss, so, lane);
             break;
         case 2:
-            spmv_item<double, uint16_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se,
+            spmv_item<VT, uint16_t, uint16_t, uint16_t, COMP>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se,
                                                                    sb, ss, so, lane);
             break;
         case 3:
-            spmv_item<double, uint32_t, uint8_t, uint8_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+            spmv_item<VT, uint32_t, uint8_t, uint8_t, COMP>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
                                                                  ss, so, lane);
             break;
         case 4:
-            spmv_item<double, uint32_t, uint8_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
+            spmv_item<VT, uint32_t, uint8_t, uint16_t, COMP>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se, sb,
                                                                   ss, so, lane);
             break;
         default:
-            spmv_item<double, uint32_t, uint16_t, uint16_t, false>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se,
+            spmv_item<VT, uint32_t, uint16_t, uint16_t, COMP>(e.A, e.B, it, e.x, e.y, alpha, beta, sa, se,
                                                                    sb, ss, so, lane);
             break;
         }
@@ -1001,14 +1004,18 @@ __global__ __launch_bounds__(kFixThreads) void k_batch_fixup(const BatchEntry *_
                                                              const uint32_t *__restrict__ hv, double alpha,
                                                              double beta, unsigned long long *cyc)
 {
-    __shared__ double sh[kFixThreads];
+    __shared__ double sh[2 * kFixThreads];
     long long t0 = 0;
     if constexpr (TIMED) t0 = clock64();
     const uint32_t mat = hv[2 * blockIdx.x];
     const BatchEntry &e = E[mat];
     const uint32_t h = hv[2 * blockIdx.x + 1];
-    fixup_strip<false>(e.A.s, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1], e.B.heavy[h * 3 + 2], e.y,
-                       alpha, beta, sh);
+    if (e.comp)  // workgroup-uniform
+        fixup_strip<true>(e.A.s, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1], e.B.heavy[h * 3 + 2], e.y,
+                          alpha, beta, sh);
+    else
+        fixup_strip<false>(e.A.s, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1], e.B.heavy[h * 3 + 2],
+                           e.y, alpha, beta, sh);
     if constexpr (TIMED) {
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1483,19 +1490,23 @@ void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int6
 
 int batch_code(int bcol_bytes, int s) { return (bcol_bytes == 4 ? 3 : 0) + (s <= 8 ? 0 : (s <= 16 ? 1 : 2)); }
 
-void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t G, const uint32_t *dheavy,
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t G64, int64_t G, const uint32_t *dheavy,
                 int64_t nheavy, double alpha, double beta, hipStream_t st, unsigned long long *cycles)
 {
-    if (G > 0) {
-        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((G + 3) / 4, 8192));
+    auto launch = [&](auto comp_tag, int64_t g0, int64_t g1) {
+        constexpr bool C = decltype(comp_tag)::value;
+        if (g1 <= g0) return;
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((g1 - g0 + 3) / 4, 8192));
         if (cycles)
-            hipLaunchKernelGGL(k_batch_spmv<true>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat, G, alpha,
-                               beta, cycles);
-        else
-            hipLaunchKernelGGL(k_batch_spmv<false>, dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat, G,
+            hipLaunchKernelGGL((k_batch_spmv<true, C>), dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat, g0, g1,
                                alpha, beta, cycles);
+        else
+            hipLaunchKernelGGL((k_batch_spmv<false, C>), dim3((unsigned)grid), dim3(256), 0, st, dE, item_mat, g0,
+                               g1, alpha, beta, cycles);
         ABHSF_HIP(hipGetLastError());
-    }
+    };
+    launch(std::integral_constant<bool, false>{}, 0, G64);
+    launch(std::integral_constant<bool, true>{}, G64, G);
     if (nheavy > 0) {
         if (cycles)
             hipLaunchKernelGGL(k_batch_fixup<true>, dim3((unsigned)nheavy), dim3(kFixThreads), 0, st, dE, dheavy,
@@ -1533,6 +1544,24 @@ void spmv_transposed(const SpmvLaunch &L, const DevMat &A, int nrhs, double scal
 #undef ABHSF_TRANS
         ABHSF_HIP(hipGetLastError());
     }
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_strided_copy(const double *__restrict__ src, int64_t lds,
+                                                      double *__restrict__ dst, int64_t ldd, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i * ldd] = src[i * lds];
+}
+}  // namespace
+
+void strided_copy(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return;
+    const int64_t nblk = (n + 255) / 256;
+    ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "vector too long");
+    hipLaunchKernelGGL(k_strided_copy, dim3((unsigned)nblk), dim3(256), 0, st, src, lds, dst, ldd, n);
+    ABHSF_HIP(hipGetLastError());
 }
 
 int max_rhs_tile(int s, bool comp)

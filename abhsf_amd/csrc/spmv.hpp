@@ -63,6 +63,7 @@ struct BatchEntry {
     double *y;
     int64_t item0;  // first global work item of this matrix
     int32_t code;   // index-type combination, batch_code()
+    int32_t comp;   // 1: FP32 values with compensated accumulation (its items follow all FP64 ones)
 };
 
 // Index-type combination of a batch member (block-column width x in-block
@@ -70,19 +71,23 @@ struct BatchEntry {
 constexpr int kBatchCodes = 6;
 int batch_code(int bcol_bytes, int s);
 
-// All matrices' work items [0, G) in one grid-stride launch (each member's
-// index-type combination, BatchEntry::code, selects its decode), then the
-// heavy-strip fix-up (dheavy: pairs {matrix, heavy index}).  item_mat[g]:
-// matrix of global item g.  FP64 storage only.  With cycles (device, [nmat],
-// zeroed by the caller) every wave adds its shader-clock cycles to its
-// matrix's counter.
-void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t G, const uint32_t *dheavy, int64_t nheavy,
-                double alpha, double beta, hipStream_t st, unsigned long long *cycles = nullptr);
+// All matrices' work items in grid-stride launches (each member's index-type
+// combination, BatchEntry::code, selects its decode): the FP64 members' items
+// [0, G64) in one launch, the FP32 members' items [G64, G) in a second one
+// (compensated accumulation), then the heavy-strip fix-up of every member
+// (dheavy: pairs {matrix, heavy index}).  item_mat[g]: matrix of global item
+// g.  With cycles (device, [nmat], zeroed by the caller) every wave adds its
+// shader-clock cycles to its matrix's counter.
+void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t G64, int64_t G, const uint32_t *dheavy,
+                int64_t nheavy, double alpha, double beta, hipStream_t st, unsigned long long *cycles = nullptr);
 
 // Symmetric storage, second pass: Y[j] += scale * T_off^T * X[j] for the
 // stored triangle T without its diagonal (global atomics; FP64 values).
 void spmv_transposed(const SpmvLaunch &L, const DevMat &A, int nrhs, double scale, const double *X, int64_t ldx,
                      double *Y, int64_t ldy, hipStream_t st);
+
+// dst[i * ldd] = src[i * lds] for i < n, enqueued on st.
+void strided_copy(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t n, hipStream_t st);
 
 // Largest power-of-two rhs tile whose s x R accumulator fits the LDS budget.
 int max_rhs_tile(int s, bool comp);

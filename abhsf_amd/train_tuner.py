@@ -231,7 +231,10 @@ def cross_validate(Xtr, ytr, ntr, nc, grid, folds=5, members=5, seed=0, log=prin
     (steps, lr) of the grid, the mean regret of the held-in model's pick on
     each held-out fold's matrices (labels ytr are log times per entry, so
     time ratios within a matrix are exp of label differences).  Returns
-    {(steps, lr): mean regret %} and the best setting."""
+    {(steps, lr): (mean regret %, its standard error)} and the choice by the
+    one-standard-error rule: the fewest steps whose mean regret is within one
+    standard error of the lowest (the simplest model the folds cannot tell
+    apart from the best)."""
     nmat = len(ntr)
     order = np.random.default_rng(seed).permutation(nmat)
     fold_of = np.empty(nmat, dtype=np.int64)
@@ -248,9 +251,12 @@ def cross_validate(Xtr, ytr, ntr, nc, grid, folds=5, members=5, seed=0, log=prin
             for i in te:
                 k = int(np.argmin(model.predict(X[i])))
                 regrets.append((math.exp(Y[i, k] - Y[i].min()) - 1.0) * 100.0)
-        res[(steps, lr)] = float(np.mean(regrets))
-        log(f"cv steps={steps} lr={lr}: mean regret {res[(steps, lr)]:.2f}%")
-    return res, min(res, key=res.get)
+        res[(steps, lr)] = (float(np.mean(regrets)), float(np.std(regrets) / math.sqrt(len(regrets))))
+        log(f"cv steps={steps} lr={lr}: mean regret {res[(steps, lr)][0]:.2f}% (se {res[(steps, lr)][1]:.2f})")
+    best = min(res, key=lambda k: res[k][0])
+    bound = res[best][0] + res[best][1]
+    ok = [k for k in res if res[k][0] <= bound]
+    return res, min(ok, key=lambda k: (k[0], res[k][0]))
 
 
 def main(argv=None):
@@ -309,7 +315,8 @@ def main(argv=None):
 
     report = {"train_loss_first": float(losses[0]), "train_loss_last": float(losses[-1]),
               "schedule": {"steps": args.steps, "lr": args.lr, "members": args.members},
-              "cv_mean_regret_pct": None if cv is None else {f"{k[0]}/{k[1]}": v for k, v in cv.items()},
+              "cv_mean_regret_pct": None if cv is None else {f"{k[0]}/{k[1]}": {"mean": v[0], "se": v[1]}
+                                                              for k, v in cv.items()},
               "train_matrices": list(ntr), "excluded_same_config": EXCLUDED_SAME_CONFIG,
               "candidates": [f"{s}/{v}" for s, v in cands], "matrices": []}
     agg = {"mlp": [], "plain_s64": [], "storage_opt": []}

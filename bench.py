@@ -443,6 +443,8 @@ def run_spmm(args, hip):
     kmax = f"{precs[0]}/k{ks[-1]}"
     out["roofline"] = roofline(out_k[kmax], f"spmm/{s}/{kmax}")
     out["per_k"] = out_k
+    if not args.no_calib:
+        out["copy_calibration"] = calib_copy_gbs(hip)
     return out
 
 
@@ -504,7 +506,7 @@ def run_batch(args, hip):
         return e0.elapsed_ms(e1) / args.steps
 
     ms = time_batch(tuned)
-    ms_plain = time_batch(plain)
+    ms_plain = float("nan") if args.batch_tuned_only else time_batch(plain)
     log(f"batched: tuned {ms:.3f} ms, plain s=64 {ms_plain:.3f} ms")
     out = base_record(args, round(2 * nnz / (ms * 1e-3) / 1e9, 2), round(ms, 4))
     out["config"] = {"workload": f"{args.batch_count} matrices, n log-uniform [1e3, 2e5], families Laplacian / "
@@ -512,12 +514,16 @@ def run_batch(args, hip):
                      "nnz": nnz, "picks": picks,
                      "tuner": (tuner.NAME + (" (throughput model)" if mpath == tuner.BATCH_MODEL_PATH else
                                              " (latency model)")) if model is not None else "fixed s=64"}
+    algo = sum(M.stats()["stored_bytes"] + 8 * M.props.n + 16 * M.props.m for M in tuned)
+    out["roofline"] = roofline({"gbs": round(algo / (ms * 1e-3) / 1e9, 1), "algo_bytes": algo}, "batch/tuned")
     out["plain_s64"] = {"gflops": round(2 * nnz / (ms_plain * 1e-3) / 1e9, 2), "ms": round(ms_plain, 4)}
     out["tuned_vs_plain"] = round(ms_plain / ms, 3)
     out["per_matrix_latency_ms"] = {"p50": round(float(np.percentile(indiv, 50)), 4),
                                     "p99": round(float(np.percentile(indiv, 99)), 4),
                                     "sum_individual": round(float(np.sum(indiv)), 3)}
     out["batched_vs_individual"] = round(float(np.sum(indiv)) / ms, 2)
+    if not args.no_calib:
+        out["copy_calibration"] = calib_copy_gbs(hip)
     return out
 
 
@@ -537,6 +543,8 @@ def main():
     ap.add_argument("--spmm-k", type=int, default=None, help="spmm: one right-hand-side count (profiler passes)")
     ap.add_argument("--spmm-prec", default=None, choices=["f64", "f32"], help="spmm: one value precision")
     ap.add_argument("--batch-count", type=int, default=2000)
+    ap.add_argument("--batch-tuned-only", action="store_true",
+                    help="batch: time only the tuned batch (profiler passes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-calib", action="store_true")

@@ -156,7 +156,10 @@ abhsf_status abhsf_spmv(abhsf_handle *h, double alpha, const double *x, double b
 
 /* Y = alpha * A * X + beta * Y for nrhs right-hand sides, X (n x nrhs) and
  * Y (m x nrhs) row-major with leading dimensions ldx, ldy >= nrhs.
- * nrhs in [1, 64].  No reference counterpart (batched multi-RHS extension). */
+ * nrhs in [1, 64].  No reference counterpart (batched multi-RHS extension).
+ * A tiled handle runs each right-hand side through its SpMV kernels (the
+ * column gathered into, and the result scattered from, contiguous vectors
+ * the handle allocates at the first such call). */
 abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const double *X, int64_t ldx,
                         double beta, double *Y, int64_t ldy);
 
@@ -193,10 +196,12 @@ abhsf_status abhsf_block_hist(const int64_t *rowptr, const int32_t *col, int64_t
 
 /* ---- Batched launch of many matrices ---------------------------------------
  * No reference counterpart.  One grid-stride launch processes the work items
- * of every matrix (each with its own block size), followed by the heavy-strip
- * fix-up; with use_graph the pair is captured once into a hipGraph and
- * replayed.  The member handles must stay alive and unchanged while the batch
- * exists.  FP64 storage only. */
+ * of every FP64 matrix (each with its own block size), a second one those of
+ * the FP32-valued members (compensated accumulation), followed by the
+ * heavy-strip fix-up of all members and, for members with symmetric or
+ * skew-symmetric storage, the mirrored triangle's pass (one launch each);
+ * with use_graph the sequence is captured once into a hipGraph and replayed.
+ * The member handles must stay alive and unchanged while the batch exists. */
 typedef struct abhsf_batch abhsf_batch;
 abhsf_status abhsf_batch_create(abhsf_handle *const *mats, int32_t count, void *stream, abhsf_batch **out);
 /* xs[i] / ys[i]: device vectors of matrix i (host arrays of device pointers). */

@@ -104,3 +104,57 @@ def test_batch_profile_cycles(gpu):
     assert c2[0] > 10 * c2[1], c2
     B2.release()
     B.release()
+
+
+def test_batch_with_fp32_and_symmetric_members(gpu):
+    """FP32-valued members (a second launch with compensated accumulation,
+    FP32 heavy-strip partials in the fix-up) and symmetric / skew-symmetric
+    members (their mirrored triangle's pass after the batched launch) next to
+    FP64 general members, eager and as replayed hipGraphs."""
+    st = hip.Stream()
+    L = synth.laplacian2d(70)
+    r, c, v = L.triplets()
+    low = c <= r
+    sym_lower = synth.from_triplets(L.m, L.n, r[low], c[low], v[low])
+    rs = synth.random_sparse(900, 900, 0.01, seed=4)
+    rr, cc, vv = rs.triplets()
+    sl = cc < rr  # strictly lower: skew-symmetric storage has no diagonal
+    skew_lower = synth.from_triplets(rs.m, rs.n, rr[sl], cc[sl], vv[sl])
+    cases = [  # (matrix, s, precision, variant, symmetry)
+        (synth.rmat(14, 16, seed=21), 64, "f32", "balanced", "general"),   # heavy strips, FP32 partials
+        (synth.block_mix(1 << 12, 5), 32, "f64", "tiled", "general"),
+        (synth.banded(5000, 20, 3), 16, "f32", "tiled", "general"),
+        (sym_lower, 8, "f64", "balanced", "symmetric"),
+        (synth.random_blocks(6000, 8, 4, 0.2, seed=2), 8, "f32", "strip", "general"),
+        (skew_lower, 32, "f64", "balanced", "skew"),
+    ]
+    Ms, xs, ys, refs = [], [], [], []
+    for i, (A, s, prec, var, sym) in enumerate(cases):
+        print(f"case batch member {i}: s={s} {prec} {var} {sym}", flush=True)
+        M = AbhsfMatrix(block_size=s, precision=prec, variant=var, stream=st)
+        M.from_csr(A.m, A.n, A.rowptr, A.col, A.val, symmetry=sym)
+        x = np.random.default_rng(i).uniform(-1, 1, A.n)
+        vals = A.val if prec == "f64" else A.val.astype(np.float32).astype(np.float64)
+        # the whole matrix: the stored triangle plus its mirror (negated for skew)
+        rr_, cc_, _ = A.triplets()
+        if sym == "general":
+            F = synth.Csr(A.m, A.n, A.rowptr, A.col, vals)
+        else:
+            off = rr_ != cc_
+            sgn = -1.0 if sym == "skew" else 1.0
+            F = synth.from_triplets(A.m, A.n, np.concatenate([rr_, cc_[off]]), np.concatenate([cc_, rr_[off]]),
+                                    np.concatenate([vals, sgn * vals[off]]))
+        Ms.append(M)
+        xs.append(DeviceArray.from_numpy(x))
+        ys.append(DeviceArray.zeros(A.m))
+        refs.append(pyoracle.csr_spmv(F.rowptr, F.col, F.val, x, 1.5, 0.0))
+    B = AbhsfBatch(Ms, st)
+    B.set_vectors(xs, ys)
+    for graph in (False, True, True):
+        for y in ys:
+            y.fill_zero(st.handle)
+        st.synchronize()
+        B.spmv_async(1.5, 0.0, graph=graph)
+        B.synchronize()
+        for i, (y, ref) in enumerate(zip(ys, refs)):
+            assert relerr(y.numpy(), ref) <= TOL, (i, graph)

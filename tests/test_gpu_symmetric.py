@@ -133,8 +133,14 @@ def test_symmetric_rejects_both_triangles_and_skew_diagonal(gpu):
     x = np.random.default_rng(0).uniform(-1, 1, T.n)
     got = run_spmv(M, T.n, x, np.zeros(T.m), 1.0, 0.0)
     assert relerr(got, pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)) <= TOL
-    with pytest.raises(AbhsfError):
-        AbhsfBatch([M])
+    # a batch member: the mirrored pass runs after the batched launch
+    B = AbhsfBatch([M])
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.zeros(T.m)
+    B.set_vectors([dx], [dy])
+    B.spmv_async(1.0, 0.0)
+    B.synchronize()
+    assert relerr(dy.numpy(), pyoracle.csr_spmv(A.rowptr, A.col, A.val, x)) <= TOL
+    B.release()
 
 
 @pytest.mark.parametrize("case", ["dense", "random-column", "mtx_general", "mtx_symmetric", "mtx_skew",

@@ -130,3 +130,26 @@ def test_u32_delta_features_match_oracle_encoder(name):
         assert round(f["ent_frac"][3] * A.nnz) == nexc, codec
         chunks = -(-A.nnz // 512)
         assert f["stored_bytes"] == 12 * A.nnz + 8 * nexc + 8 * chunks + 4 * (chunks + 1)
+
+
+def test_sampled_delta_features_estimate_exact_counts():
+    """Above SAMPLE_NNZ entries the delta features come from a weighted sample
+    of row bands: on an R-MAT of 4e6 entries the estimated word classes and
+    exception counts are close to the exact counts of the oracle encoders."""
+    from oracle import pyoracle
+    A = synth.rmat(18, 16, seed=3)
+    assert A.nnz > tuner.SAMPLE_NNZ
+    smp = tuner._band_sample(A.rowptr, A.col)
+    assert smp[0].size < 0.8 * A.nnz  # a sample, not every element
+    # varint word classes against the exact deltas of row-major keys
+    K = max(1, tuner._ceil_log2(A.n))
+    r, c, _ = A.triplets()
+    keys = (r.astype(np.uint64) << np.uint64(K)) + c.astype(np.uint64)
+    d = np.diff(keys, prepend=np.uint64(0))
+    exact = np.array([np.mean(d < 64), np.mean((d >= 64) & (d < 1 << 14)), np.mean((d >= 1 << 14) & (d < 1 << 30))])
+    est = np.array(tuner.delta_features(A.rowptr, A.col, A.n, "varint", smp)["ent_frac"][:3])
+    assert np.all(np.abs(est - exact) < 0.05), (est, exact)
+    for codec, layout in (("u32_row", pyoracle.DELTA32_ROWMAJOR), ("u32_block", pyoracle.DELTA32_CACHEBLOCK)):
+        nexc = pyoracle.delta32_encode(layout, A.m, A.rowptr, A.col)["exc"].size
+        f = tuner.delta_features(A.rowptr, A.col, A.n, codec, smp)
+        assert abs(f["ent_frac"][3] * A.nnz - nexc) <= 0.35 * nexc + 50, (codec, f["ent_frac"][3] * A.nnz, nexc)
