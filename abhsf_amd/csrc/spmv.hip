@@ -933,7 +933,10 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
 // (R-MAT scale 22 at s = 256: the heaviest strip holds 584 units and sets the
 // pass's length.  SpMV 0.545 ms with 512 threads x 4 chains, 0.532 ms with
 // 256 x 16; 256 x 8 and 512 x 16 are in between, 32 chains spill: 0.61+ ms.
-// 1024 threads x 4 chains was slower than 512 x 4.)
+// 1024 threads x 4 chains was slower than 512 x 4.)  With compensated
+// partials (two loads and a TwoSum per chain step) 16 chains do not stay in
+// registers and the loads serialise: fix-up 285 us with 16 chains, 77 us with
+// 8, 119 us with 4 (FP64 partials: 38 / 58 / 82 us).
 #ifndef ABHSF_FIX_THREADS
 #define ABHSF_FIX_THREADS 256
 #endif
@@ -942,7 +945,8 @@ static_assert(kFixThreads >= 256, "at least one thread per row of a 256-row stri
 #ifndef ABHSF_FIX_CHAINS
 #define ABHSF_FIX_CHAINS 16
 #endif
-constexpr int kFixChains = ABHSF_FIX_CHAINS;
+template <bool COMP>
+constexpr int kFixChains = COMP ? ABHSF_FIX_CHAINS / 2 : ABHSF_FIX_CHAINS;
 
 template <bool COMP>
 __device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
@@ -965,13 +969,13 @@ __device__ inline void fixup_strip(int s, int64_t m, int64_t I, const double *__
     const int r = (int)(threadIdx.x / L);
     const uint32_t l = threadIdx.x % L;
     const size_t stride = (size_t)s * (COMP ? 2 : 1);
-    double hi[kFixChains], lo[kFixChains];
+    double hi[kFixChains<COMP>], lo[kFixChains<COMP>];
 #pragma unroll
-    for (int j = 0; j < kFixChains; j++) hi[j] = lo[j] = 0.0;
+    for (int j = 0; j < kFixChains<COMP>; j++) hi[j] = lo[j] = 0.0;
     uint32_t u = l;
-    for (; u + (uint32_t)(kFixChains - 1) * L < nu; u += (uint32_t)kFixChains * L) {
+    for (; u + (uint32_t)(kFixChains<COMP> - 1) * L < nu; u += (uint32_t)kFixChains<COMP> * L) {
 #pragma unroll
-        for (int j = 0; j < kFixChains; j++) {
+        for (int j = 0; j < kFixChains<COMP>; j++) {
             const double *p = part + (size_t)(u0 + u + (uint32_t)j * L) * stride;
             fix_add<COMP>(hi[j], lo[j], p[r], COMP ? p[s + r] : 0.0);
         }
@@ -981,7 +985,7 @@ __device__ inline void fixup_strip(int s, int64_t m, int64_t I, const double *__
         fix_add<COMP>(hi[0], lo[0], p[r], COMP ? p[s + r] : 0.0);
     }
 #pragma unroll
-    for (int j = 1; j < kFixChains; j++) fix_add<COMP>(hi[0], lo[0], hi[j], lo[j]);
+    for (int j = 1; j < kFixChains<COMP>; j++) fix_add<COMP>(hi[0], lo[0], hi[j], lo[j]);
     sh[threadIdx.x] = hi[0];
     if constexpr (COMP) sh[kFixThreads + threadIdx.x] = lo[0];
     __syncthreads();
@@ -1077,16 +1081,187 @@ __device__ inline void dense_block_mfma(const VT *__restrict__ D, int s, int64_t
     }
 }
 
-// Rows per lane and step in the SpMM kernel's CSR group (their loads are in
-// flight together).  A block takes s * R / (64 * kSpmmRows) steps, each a
-// chain of dependent loads (row pointers, columns, X), so with one row per
-// lane the time grew with R.  27-point stencil 160^3, s = 64, FP64: k = 8
-// 1.542 -> 1.388 ms, k = 32 6.51 -> 5.86 ms with 2 rows; 4 and 8 rows were
-// slower than 1 (k = 8: 1.70 and 2.42 ms).
-#ifndef ABHSF_SPMM_ROWS
-#define ABHSF_SPMM_ROWS 2
-#endif
-constexpr int kSpmmRows = ABHSF_SPMM_ROWS;
+// CSR group of the SpMM kernel, strip I.  Lanes own block rows as in the
+// single-vector CSR walk (for s <= 64, lane l takes row l % s of block l / s
+// of each step; for s > 64, one block per step in s / 64 row passes) and hold
+// the row's R right-hand sides in registers: each entry's column and value
+// are loaded once and its R values of X are read back to back (16-byte loads
+// where X allows).  Row sums carry over the strip's blocks; the G = 64 / s
+// lanes that share a row (s < 64) are summed with xor-shuffles, and the row's
+// owner adds the sums to the accumulator (one writer per element: no atomics).
+// X holds the wave's R columns (nrhs of them valid) at stride ldx.
+// 27-point stencil 160^3, s = 64, FP64, k = 8 / k = 32: 1.10 / 4.07 ms, against
+// 1.39 / 4.94 ms for a walk of 8 rows of R lanes per step (s * R / 64 chains
+// of dependent loads per block).  More loads in flight per lane were slower
+// throughout: two blocks per lane 1.43 / 5.87 ms, 2 or 4 entries per chain
+// 1.38 / 5.10 and 1.62 / 6.04 ms, the next block's row pointers prefetched
+// 1.22 / 4.52 ms -- the X reads queue in the memory pipeline (address unit
+// busy 65 % of the time), not in each wave's dependency chain.
+template <typename VT, typename BT, typename RT, int R, bool COMP>
+__device__ inline void spmm_csr_rows(const DevMat &A, int64_t I, const double *__restrict__ X, int64_t ldx,
+                                     int nrhs, const Acc &acc, uint32_t *s_st, uint32_t *s_bc, int lane)
+{
+    const uint32_t b0 = A.sptr[CSR][I], b1 = A.sptr[CSR][I + 1];
+    if (b0 >= b1) return;
+    uint32_t e = A.eptr[CSR][I];
+    const int s = A.s;
+    const int sw = s < kWave ? s : kWave;  // lanes per block
+    const int G = kWave / sw;              // blocks per wave step
+    const int passes = s / sw;             // row passes per block
+    const int lb = lane / sw, lr = lane % sw;
+    const int nr = nrhs < R ? nrhs : R;
+    // whole tile, X rows 16-byte aligned: R / 2 double2 loads per entry
+    const bool vec = nr == R && ((reinterpret_cast<uintptr_t>(X) & 15u) == 0) && ((ldx & 1) == 0);
+    const VT *__restrict__ val = static_cast<const VT *>(A.val[CSR]);
+    const uint8_t *__restrict__ ccol = A.csr_col;
+    const RT *__restrict__ rowptr = static_cast<const RT *>(A.csr_rowptr);
+#pragma unroll 1
+    for (uint32_t cb = b0; cb < b1; cb += kWave) {
+        const uint32_t jb = cb + lane;
+        const bool ok = jb < b1;
+        const uint32_t z = ok ? (uint32_t)A.z1[CSR][jb] + 1u : 0u;
+        const uint32_t incl = wave_incl_scan_dpp(z);
+        s_st[lane] = e + incl - z;
+        s_bc[lane] = ok ? (uint32_t)ld<BT>(A.bcol[CSR], jb) : 0u;
+        e += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        wave_sync();
+        const int nbk = (int)min<uint32_t>(kWave, b1 - cb);
+#pragma unroll 1
+        for (int ps = 0; ps < passes; ps++) {
+            const int r = ps * sw + lr;
+            double hi[R], lo[R];
+#pragma unroll
+            for (int jj = 0; jj < R; jj++) hi[jj] = lo[jj] = 0.0;
+#pragma unroll 1
+            for (int j0 = 0; j0 < nbk; j0 += G) {
+                const int jblk = j0 + lb;
+                const bool h = jblk < nbk;
+                const RT *rp = rowptr + (uint64_t)(cb + (uint32_t)(h ? jblk : 0)) * (uint64_t)(s + 1);
+                const uint32_t ra = h ? (uint32_t)rp[r] : 0u, rc = h ? (uint32_t)rp[r + 1] : 0u;
+                const uint32_t q0 = h ? s_st[jblk] + ra : 0u, ql = rc - ra;
+                const int64_t xc = h ? (int64_t)s_bc[jblk] * s : 0;
+                // (the loads below carry the t < ql predicate although it holds
+                // inside the loop: that form measured 5 % faster, k = 32 4.07
+                // against 4.27 ms)
+                const uint32_t len = max(0u, ql);
+#pragma unroll 1
+                for (uint32_t t = 0; t < len; t++) {
+                    double v, xv[R];
+                    int64_t xr;
+                    const bool hh = t < ql;
+                    xr = hh ? (xc + ccol[q0 + t]) * ldx : 0;
+                    v = hh ? (double)val[q0 + t] : 0.0;
+                    bool done = false;
+                    if constexpr (R >= 2) {
+                        if (vec) {  // 16-byte loads: half the address work per value
+                            const double2 *xp = reinterpret_cast<const double2 *>(X + xr);
+#pragma unroll
+                            for (int jj = 0; jj < R; jj += 2) {
+                                double2 w = {0.0, 0.0};
+                                if (hh) w = xp[jj >> 1];
+                                xv[jj] = w.x;
+                                xv[jj + 1] = w.y;
+                            }
+                            done = true;
+                        }
+                    }
+                    if (!done) {
+#pragma unroll
+                        for (int jj = 0; jj < R; jj++) xv[jj] = (hh && jj < nr) ? X[xr + jj] : 0.0;
+                    }
+                    if (hh) {
+#pragma unroll
+                        for (int jj = 0; jj < R; jj++) acc_add<COMP>(hi[jj], lo[jj], v, xv[jj]);
+                    }
+                }
+            }
+            for (int o = sw; o < kWave; o <<= 1) {
+#pragma unroll
+                for (int jj = 0; jj < R; jj++) {
+                    const double q = __shfl_xor(hi[jj], o, kWave);
+                    if constexpr (COMP) {
+                        const double qe = __shfl_xor(lo[jj], o, kWave);
+                        double sum, err;
+                        two_sum(hi[jj], q, sum, err);
+                        hi[jj] = sum;
+                        lo[jj] += qe + err;
+                    } else {
+                        hi[jj] += q;
+                    }
+                }
+            }
+            if (lb == 0) {
+#pragma unroll
+                for (int jj = 0; jj < R; jj++) {
+                    if (jj >= nr) break;
+                    const uint32_t ai = (uint32_t)r * R + jj;
+                    if constexpr (COMP) {
+                        double sum, err;
+                        two_sum(acc.hi[ai], hi[jj], sum, err);
+                        acc.hi[ai] = sum;
+                        acc.lo[ai] += err + lo[jj];
+                    } else {
+                        acc.hi[ai] += hi[jj];
+                    }
+                }
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Products of a decoded window of nq <= 64 entries (rows d_row, X rows
+// d_col, values d_val, in LDS) with the wave's R columns of X, added to the
+// strip accumulator with LDS atomics.  With a whole, 16-byte aligned tile
+// (vec) H = R / 2 lanes take an entry, a double2 of X each, so H steps of
+// 64 / H entries cover the window and all of its X loads are issued before
+// its adds (one memory round trip per window; R-lane steps of one column per
+// lane, below, wait for each step's loads in turn).
+template <int R, bool COMP>
+__device__ inline void spmm_window(uint32_t nq, const uint32_t *d_row, const uint32_t *d_col, const double *d_val,
+                                   const double *__restrict__ X, int64_t ldx, int nrhs, bool vec, const Acc &acc,
+                                   int lane)
+{
+    if constexpr (R >= 2) {
+        if (vec) {
+            constexpr int H = R / 2, E2 = kWave / H;
+            const int slot2 = lane / H, c2 = lane % H;
+            double2 xw[H];
+#pragma unroll
+            for (int g = 0; g < H; g++) {
+                const uint32_t ent = (uint32_t)(g * E2 + slot2);
+                xw[g] = double2{0.0, 0.0};
+                if (ent < nq) xw[g] = reinterpret_cast<const double2 *>(X + (int64_t)d_col[ent] * ldx)[c2];
+            }
+#pragma unroll
+            for (int g = 0; g < H; g++) {
+                const uint32_t ent = (uint32_t)(g * E2 + slot2);
+                if (ent < nq) {
+                    const double v = d_val[ent];
+                    const uint32_t ai = d_row[ent] * R + 2 * c2;
+                    double pe;
+                    double p = prod<COMP>(v, xw[g].x, pe);
+                    acc_atomic<COMP>(acc, ai, p, pe);
+                    p = prod<COMP>(v, xw[g].y, pe);
+                    acc_atomic<COMP>(acc, ai + 1, p, pe);
+                }
+            }
+            return;
+        }
+    }
+    constexpr int E = kWave / R;  // entries per step
+    const int slot = lane / R, j = lane % R;
+#pragma unroll 1
+    for (uint32_t g = 0; g < nq; g += E) {
+        const uint32_t ent = g + slot;
+        if (ent < nq && j < nrhs) {
+            const double xv = X[(int64_t)d_col[ent] * ldx + j];
+            double pe;
+            const double p = prod<COMP>(d_val[ent], xv, pe);
+            acc_atomic<COMP>(acc, d_row[ent] * R + j, p, pe);
+        }
+    }
+}
 
 // Two launch shapes: sib == 0, wave w of workgroup b takes strip b * wpb + w
 // for the R right-hand sides at X, Y; sib > 0, the workgroup's sib waves all
@@ -1115,6 +1290,8 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
     }
     const bool jok = j < nrhs;
     if (I >= nstrip_total) return;  // whole wave leaves; no workgroup barriers below
+    // whole tile, X rows 16-byte aligned: the decoded windows read X as double2
+    const bool vec = nrhs >= R && ((reinterpret_cast<uintptr_t>(X) & 15u) == 0) && ((ldx & 1) == 0);
 
     const size_t per_wave = spmm_wave_lds(s, R, COMP);
     unsigned char *base = smem + per_wave * wave;
@@ -1169,91 +1346,16 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
                     d_val[lane] = (double)val[e + t];
                 }
                 wave_sync();
-                const uint32_t nq = min<uint32_t>(kWave, T - q);
-#pragma unroll 1
-                for (uint32_t g = 0; g < nq; g += E) {
-                    const uint32_t ent = g + slot;
-                    if (ent < nq && jok) {
-                        const double xv = X[(int64_t)d_col[ent] * ldx + j];
-                        double pe;
-                        const double p = prod<COMP>(d_val[ent], xv, pe);
-                        acc_atomic<COMP>(acc, d_row[ent] * R + j, p, pe);
-                    }
-                }
+                spmm_window<R, COMP>(min<uint32_t>(kWave, T - q), d_row, d_col, d_val, X, ldx, nrhs, vec, acc, lane);
                 wave_sync();
             }
             e += T;
         }
     }
 
-    // ---- CSR group: one block at a time, E rows per step, R lanes per row
-    // (one per right-hand side); a lane walks its row's entries and adds the
-    // row sum to the accumulator (rows of one step are distinct: no atomics)
-    {
-        const uint32_t b0 = A.sptr[CSR][I], b1 = A.sptr[CSR][I + 1];
-        uint64_t e = A.eptr[CSR][I];
-        const VT *val = static_cast<const VT *>(A.val[CSR]);
-        const RT *rowptr = static_cast<const RT *>(A.csr_rowptr);
-#pragma unroll 1
-        for (uint32_t b = b0; b < b1; b++) {
-            const int64_t c0 = (int64_t)ld<BT>(A.bcol[CSR], b) * s;
-            const RT *rp = rowptr + (uint64_t)b * (uint64_t)(s + 1);
-#pragma unroll 1
-            for (int r0 = 0; r0 < s; r0 += E * kSpmmRows) {
-                // kSpmmRows rows per lane (r0 + u*E + slot), walked together so
-                // that their loads are in flight at once; each row's entries
-                // are added in order
-                uint32_t qb[kSpmmRows], ql[kSpmmRows], len = 0;
-#pragma unroll
-                for (int u = 0; u < kSpmmRows; u++) {
-                    const int r = r0 + u * E + slot;
-                    const bool ok = r < s && jok;
-                    qb[u] = ok ? (uint32_t)rp[r] : 0u;
-                    ql[u] = ok ? (uint32_t)rp[r + 1] - qb[u] : 0u;
-                    len = max(len, ql[u]);
-                }
-                double hi[kSpmmRows], lo[kSpmmRows];
-#pragma unroll
-                for (int u = 0; u < kSpmmRows; u++) hi[u] = lo[u] = 0.0;
-#pragma unroll 1
-                for (uint32_t t = 0; t < len; t++) {
-                    // all rows' column and value loads, then their x loads,
-                    // then the sums
-                    uint32_t cc[kSpmmRows];
-                    double v[kSpmmRows], xv[kSpmmRows];
-#pragma unroll
-                    for (int u = 0; u < kSpmmRows; u++) {
-                        const bool h = t < ql[u];
-                        const uint64_t q = e + qb[u] + t;
-                        cc[u] = h ? (uint32_t)A.csr_col[q] : 0u;
-                        v[u] = h ? (double)val[q] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < kSpmmRows; u++)
-                        xv[u] = (t < ql[u]) ? X[(c0 + cc[u]) * ldx + j] : 0.0;
-#pragma unroll
-                    for (int u = 0; u < kSpmmRows; u++)
-                        if (t < ql[u]) acc_add<COMP>(hi[u], lo[u], v[u], xv[u]);
-                }
-#pragma unroll
-                for (int u = 0; u < kSpmmRows; u++)
-                    if (ql[u]) {
-                        const uint32_t ai = (uint32_t)(r0 + u * E + slot) * R + j;
-                        if constexpr (COMP) {
-                            double sum, err;
-                            two_sum(acc.hi[ai], hi[u], sum, err);
-                            acc.hi[ai] = sum;
-                            acc.lo[ai] += err + lo[u];
-                        } else {
-                            acc.hi[ai] += hi[u];
-                        }
-                    }
-            }
-            wave_sync();
-            e += (uint64_t)A.z1[CSR][b] + 1u;
-        }
-    }
-
+    // ---- CSR group: lanes own block rows, R right-hand sides per lane in
+    // registers across the strip's blocks
+    spmm_csr_rows<VT, BT, RT, R, COMP>(A, I, X, ldx, nrhs, acc, s_end, s_bc, lane);
     // ---- bitmap and dense groups: position-parallel over 64-bit words --------
     // Within a word, positions t*E + slot (t < R) share a row in aligned groups
     // of G = min(E, s) slots, so products are summed with xor-shuffles over the
@@ -1295,16 +1397,7 @@ __global__ __launch_bounds__(256) void k_spmm_strip(DevMat A, const double *__re
                             d_val[ix] = (double)val[vbase + ix];
                         }
                         wave_sync();
-#pragma unroll 1
-                        for (uint32_t g = 0; g < nq; g += E) {
-                            const uint32_t ent = g + slot;
-                            if (ent < nq && jok) {
-                                const double xv = X[(int64_t)d_col[ent] * ldx + j];
-                                double pe;
-                                const double p = prod<COMP>(d_val[ent], xv, pe);
-                                acc_atomic<COMP>(acc, d_row[ent] * R + j, p, pe);
-                            }
-                        }
+                        spmm_window<R, COMP>(nq, d_row, d_col, d_val, X, ldx, nrhs, vec, acc, lane);
                         wave_sync();
                         vbase += nq;
                     }

@@ -181,6 +181,31 @@ def measure_candidates(A, cands, args, hip, stream, check=True):
     return res, x
 
 
+def measure_fp32(A, s, variant, args, hip, stream, x, f64_ms):
+    """FP32 value storage with compensated (TwoSum) accumulation at (s,
+    variant): time per SpMV and the max-norm relative error against the FP64
+    product of the FP64 values and of the f32-rounded values."""
+    from abhsf_amd import AbhsfMatrix
+    dx, dy = hip.DeviceArray.from_numpy(x), hip.DeviceArray.zeros(A.m)
+    d_rp, d_c, d_v = (hip.DeviceArray.from_numpy(a) for a in (A.rowptr, A.col, A.val))
+    M = AbhsfMatrix(block_size=s, variant=variant, precision="f32", stream=stream)
+    M.from_csr(A.m, A.n, d_rp, d_c, d_v, on_device=True, nnz=A.nnz)
+    M.spmv_async(1.0, dx, 0.0, dy)
+    stream.synchronize()
+    got = dy.numpy()
+    ref64 = csr_spmv_numpy(A, x)
+    rows = np.repeat(np.arange(A.m), np.diff(A.rowptr))
+    ref32 = np.bincount(rows, weights=A.val.astype(np.float32).astype(np.float64) * x[A.col], minlength=A.m)
+    dy.fill_zero(stream.handle)
+    ms = time_spmv(M, hip, dx, dy, args.steps, args.warmup, stream) / args.steps
+    st = M.stats()
+    M.release()
+    return {"s": s, "variant": variant, "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2), "ms": round(ms, 4),
+            "stored_bytes": st["stored_bytes"], "time_vs_f64": round(ms / f64_ms, 3),
+            "maxnorm_rel_err_vs_f64_values": float(np.max(np.abs(got - ref64)) / np.max(np.abs(ref64))),
+            "maxnorm_rel_err_vs_f32_values": float(np.max(np.abs(got - ref32)) / np.max(np.abs(ref32)))}
+
+
 def tuned_pick(A, allowed_s, hip):
     from abhsf_amd import tuner
     if not os.path.exists(tuner.MODEL_PATH):
@@ -382,6 +407,9 @@ def run_rmat(args, hip):
     out["per_candidate"] = {f"{s}/{v}": r for (s, v), r in res.items()}
     if not args.no_calib:
         out["copy_calibration"] = calib_copy_gbs(hip)
+    if not args.no_extra and args.variant is None and pick[0] > 0:
+        log("fp32 values at the pick ...")
+        out["fp32_storage"] = measure_fp32(A, pick[0], pick[1], args, hip, stream, x, best["ms"])
     if not args.no_extra and args.variant is None:
         log("block_mix n=2^22 ...")
         bm = run_block_mix(argparse.Namespace(**{**vars(args), "no_cpu_baseline": True, "no_calib": True,
