@@ -923,9 +923,9 @@ __global__ __launch_bounds__(256) void k_batch_spmv(const BatchEntry *__restrict
     }
 }
 
-// Rows of one heavy strip I: y = alpha * (sum of the partial rows of units
-// [u0, u0 + nu)) + beta * y, by a workgroup of kFixThreads threads.  The
-// L = kFixThreads / s threads of row r take units l, l + L, l + 2L, ... in
+// Rows [r0, r0 + rs) of one heavy strip I: y = alpha * (sum of the partial
+// rows of units [u0, u0 + nu)) + beta * y, by a workgroup of kFixThreads
+// threads.  The L = kFixThreads / rs threads of row r take units l, l + L, l + 2L, ... in
 // kFixChains interleaved chains (that many loads in flight per thread); the
 // chains and then the L threads are combined in a fixed order, so the sum
 // does not depend on timing.  Partials of unit u: part[u * s * (COMP ? 2 : 1)
@@ -947,6 +947,13 @@ static_assert(kFixThreads >= 256, "at least one thread per row of a 256-row stri
 #endif
 template <bool COMP>
 constexpr int kFixChains = COMP ? ABHSF_FIX_CHAINS / 2 : ABHSF_FIX_CHAINS;
+// Rows per fix-up workgroup: a strip of s > kFixRows rows is summed by s /
+// kFixRows workgroups, kFixThreads / kFixRows threads per row.  R-MAT scale
+// 22 at s = 256 (heaviest strip 585 units), fix-up with 256 / 128 / 64-row
+// slices: FP64 39 / 31.5 / 36 us, compensated 91 / 48 / 40 us.
+template <bool COMP>
+constexpr int kFixRows = COMP ? 64 : 128;
+static_assert(kFixThreads % 128 == 0, "whole threads per row");
 
 template <bool COMP>
 __device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
@@ -962,12 +969,14 @@ __device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
 }
 
 template <bool COMP>
-__device__ inline void fixup_strip(int s, int64_t m, int64_t I, const double *__restrict__ part, uint32_t u0,
-                                   uint32_t nu, double *__restrict__ y, double alpha, double beta, double *sh)
+__device__ inline void fixup_strip(int s, int rs, int r0, int64_t m, int64_t I, const double *__restrict__ part,
+                                   uint32_t u0, uint32_t nu, double *__restrict__ y, double alpha, double beta,
+                                   double *sh)
 {
-    const uint32_t L = (uint32_t)(kFixThreads / s);
-    const int r = (int)(threadIdx.x / L);
-    const uint32_t l = threadIdx.x % L;
+    const uint32_t L = (uint32_t)(kFixThreads / rs);
+    // consecutive threads take consecutive rows of one unit (coalesced loads)
+    const int r = r0 + (int)(threadIdx.x % (uint32_t)rs);
+    const uint32_t l = threadIdx.x / (uint32_t)rs;
     const size_t stride = (size_t)s * (COMP ? 2 : 1);
     double hi[kFixChains<COMP>], lo[kFixChains<COMP>];
 #pragma unroll
@@ -990,9 +999,10 @@ __device__ inline void fixup_strip(int s, int64_t m, int64_t I, const double *__
     if constexpr (COMP) sh[kFixThreads + threadIdx.x] = lo[0];
     __syncthreads();
     if (l == 0) {
+        const uint32_t lr = (uint32_t)(r - r0);
         double H = 0.0, Lo = 0.0;
         for (uint32_t t = 0; t < L; t++)
-            fix_add<COMP>(H, Lo, sh[(uint32_t)r * L + t], COMP ? sh[kFixThreads + (uint32_t)r * L + t] : 0.0);
+            fix_add<COMP>(H, Lo, sh[t * (uint32_t)rs + lr], COMP ? sh[kFixThreads + t * (uint32_t)rs + lr] : 0.0);
         const int64_t gr = I * s + r;
         if (gr < m) {
             const double a = H + Lo;
@@ -1015,11 +1025,11 @@ __global__ __launch_bounds__(kFixThreads) void k_batch_fixup(const BatchEntry *_
     const BatchEntry &e = E[mat];
     const uint32_t h = hv[2 * blockIdx.x + 1];
     if (e.comp)  // workgroup-uniform
-        fixup_strip<true>(e.A.s, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1], e.B.heavy[h * 3 + 2], e.y,
-                          alpha, beta, sh);
+        fixup_strip<true>(e.A.s, e.A.s, 0, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1],
+                          e.B.heavy[h * 3 + 2], e.y, alpha, beta, sh);
     else
-        fixup_strip<false>(e.A.s, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1], e.B.heavy[h * 3 + 2],
-                           e.y, alpha, beta, sh);
+        fixup_strip<false>(e.A.s, e.A.s, 0, e.A.m, e.B.heavy[h * 3], e.B.part, e.B.heavy[h * 3 + 1],
+                           e.B.heavy[h * 3 + 2], e.y, alpha, beta, sh);
     if constexpr (TIMED) {
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1033,8 +1043,12 @@ __global__ __launch_bounds__(kFixThreads) void k_spmv_fixup(int s, int64_t m, do
                                                             double beta, Balance B)
 {
     __shared__ double sh[kFixThreads * (COMP ? 2 : 1)];
-    const int64_t h = blockIdx.x;
-    fixup_strip<COMP>(s, m, B.heavy[h * 3], B.part, B.heavy[h * 3 + 1], B.heavy[h * 3 + 2], y, alpha, beta, sh);
+    // strips of more than kFixRows rows: one workgroup per kFixRows-row slice
+    const int rs = s < kFixRows<COMP> ? s : kFixRows<COMP>, slices = s / rs;
+    const int64_t h = blockIdx.x / slices;
+    const int r0 = (int)(blockIdx.x % slices) * rs;
+    fixup_strip<COMP>(s, rs, r0, m, B.heavy[h * 3], B.part, B.heavy[h * 3 + 1], B.heavy[h * 3 + 2], y, alpha, beta,
+                      sh);
 }
 
 // LDS bytes per wave of the SpMM kernel: the s x R accumulator (twice when
@@ -1232,6 +1246,35 @@ __device__ inline void spmm_window(uint32_t nq, const uint32_t *d_row, const uin
                 const uint32_t ent = (uint32_t)(g * E2 + slot2);
                 xw[g] = double2{0.0, 0.0};
                 if (ent < nq) xw[g] = reinterpret_cast<const double2 *>(X + (int64_t)d_col[ent] * ldx)[c2];
+            }
+            if constexpr (COMP) {
+                // every returning add of the window first, then the TwoSums and
+                // the compensation adds: the hi and lo arrays each receive their
+                // adds in the same order as one product at a time, so the sums
+                // are the same, but the returns are awaited once
+                double p[2 * H], pe[2 * H], old[2 * H];
+                uint32_t ai[2 * H];
+#pragma unroll
+                for (int g = 0; g < H; g++) {
+                    const uint32_t ent = (uint32_t)(g * E2 + slot2);
+                    const bool ok = ent < nq;
+                    const double v = ok ? d_val[ent] : 0.0;
+                    ai[2 * g] = ok ? d_row[ent] * R + 2 * c2 : 0u;
+                    ai[2 * g + 1] = ai[2 * g] + 1;
+                    p[2 * g] = prod<true>(v, xw[g].x, pe[2 * g]);
+                    p[2 * g + 1] = prod<true>(v, xw[g].y, pe[2 * g + 1]);
+                }
+#pragma unroll
+                for (int i = 0; i < 2 * H; i++)
+                    if ((uint32_t)((i >> 1) * E2 + slot2) < nq) old[i] = atomicAdd(&acc.hi[ai[i]], p[i]);
+#pragma unroll
+                for (int i = 0; i < 2 * H; i++)
+                    if ((uint32_t)((i >> 1) * E2 + slot2) < nq) {
+                        double sum, err;
+                        two_sum(old[i], p[i], sum, err);
+                        atomicAdd(&acc.lo[ai[i]], err + pe[i]);
+                    }
+                return;
             }
 #pragma unroll
             for (int g = 0; g < H; g++) {
@@ -1534,8 +1577,10 @@ void launch_spmv(const DevMat &A, const Balance &B, bool coo_single, const doubl
         launch_strip<VT, BT, IT, RT, COMP, false>(A, b, nblk, coo_single, x, y, alpha, beta, st);
     ABHSF_HIP(hipGetLastError());
     if (b.n_heavy) {
-        hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)b.n_heavy), dim3(kFixThreads), 0, st, A.s,
-                           A.m, y, alpha, beta, b);
+        const int64_t slices = A.s > kFixRows<COMP> ? A.s / kFixRows<COMP> : 1;
+        ABHSF_REQUIRE(b.n_heavy * slices < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many heavy strips");
+        hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)(b.n_heavy * slices)), dim3(kFixThreads), 0, st,
+                           A.s, A.m, y, alpha, beta, b);
         ABHSF_HIP(hipGetLastError());
     }
 }
