@@ -408,6 +408,34 @@ def test_panel_variant(gpu, name, s):
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("s", [16, 64, 256])
+@pytest.mark.parametrize("case", ["stencil27_24", "block_mix_2048"])
+def test_spmm_x_alignment(gpu, case, s, precision):
+    """SpMM reads whole 16-byte-aligned column tiles of X as double2 and
+    anything else one double at a time: X at a 16-byte boundary and X one
+    double past it (same ldx) must both give the oracle's product exactly
+    (integer values and X keep every sum exact, whatever the order)."""
+    A = synth.stencil27(24) if case == "stencil27_24" else synth.block_mix(2048, seed=3)
+    print(f"case spmm x alignment {case} s={s} {precision}", flush=True)
+    rng = np.random.default_rng(s)
+    vi = (rng.integers(1, 9, A.nnz) * rng.choice([-1.0, 1.0], A.nnz)).astype(np.float64)
+    M = AbhsfMatrix(block_size=s, precision=precision)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, vi)
+    for nrhs in (8, 16):
+        X = rng.integers(-8, 9, (A.n, nrhs)).astype(np.float64)
+        ref = np.stack([pyoracle.csr_spmv(A.rowptr, A.col, vi, X[:, j].copy()) for j in range(nrhs)], axis=1)
+        flat = np.concatenate([[0.0], X.ravel()])
+        buf = DeviceArray.from_numpy(flat)
+        dX = DeviceArray.from_numpy(X)
+        assert dX.address % 16 == 0 and (buf.address + 8) % 16 == 8
+        for xaddr in (dX.address, buf.address + 8):
+            dY = DeviceArray.zeros((A.m, nrhs))
+            M.spmm_async(nrhs, 1.0, xaddr, nrhs, 0.0, dY, nrhs)
+            M.synchronize()
+            assert np.array_equal(dY.numpy(), ref), (nrhs, xaddr % 16)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("s", [16, 32, 64, 128])
 def test_unaligned_x(gpu, precision, s):
     """x may start at any double (here x + 1 of a device array, 8 bytes past a
