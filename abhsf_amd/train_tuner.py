@@ -263,8 +263,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     # latency defaults: the choice of a 5-fold cross-validation over the
-    # training matrices (--cv reruns it); the batch set is 8x larger and
-    # keeps the longer schedule
+    # training matrices with the one-standard-error rule (--cv reruns it);
+    # the batch set is 8x larger and keeps the longer schedule
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--lr", type=float, default=None)
     ap.add_argument("--members", type=int, default=5)
@@ -275,7 +275,7 @@ def main(argv=None):
                     help="latency: retrain from a saved tuner_dataset.npz (its labels) instead of sweeping")
     args = ap.parse_args(argv)
     if args.steps is None:
-        args.steps = 4000 if args.mode == "batch" else 700
+        args.steps = 4000 if args.mode == "batch" else 1500
     if args.lr is None:
         args.lr = 2e-3 if args.mode == "batch" else 1e-2
     os.makedirs(args.out, exist_ok=True)
