@@ -114,6 +114,10 @@ U32_MATS = {
     "block_mix_2^13": lambda: synth.block_mix(1 << 13, 4),
     "wide_30x200000": lambda: synth.random_sparse(30, 200000, 0.001, seed=7),
     "sparse_rows": lambda: synth.from_triplets(5000, 700, [0, 0, 1999, 4999], [0, 699, 5, 3], [1.0, 2.0, 3.0, 4.0]),
+    # one entry per row: a 64-element step spans 8 row bands, so cache-block
+    # keys take both the per-wave band bins and the global adds past them
+    "diagonal_3000": lambda: synth.from_triplets(3000, 3000, np.arange(3000), np.arange(3000),
+                                                 np.arange(1, 3001, dtype=np.float64)),
     "empty": lambda: synth.from_triplets(64, 64, [], [], []),
 }
 
