@@ -157,6 +157,8 @@ abhsf_status abhsf_spmv(abhsf_handle *h, double alpha, const double *x, double b
 /* Y = alpha * A * X + beta * Y for nrhs right-hand sides, X (n x nrhs) and
  * Y (m x nrhs) row-major with leading dimensions ldx, ldy >= nrhs.
  * nrhs in [1, 64].  No reference counterpart (batched multi-RHS extension).
+ * Any alignment of X is accepted; X at a 16-byte boundary with an even ldx
+ * reads whole column tiles in 16-byte loads (faster).
  * A tiled handle runs each right-hand side through its SpMV kernels (the
  * column gathered into, and the result scattered from, contiguous vectors
  * the handle allocates at the first such call). */
