@@ -310,10 +310,12 @@ def test_repeated_spmv_bitwise_identical(gpu, variant):
 
 @pytest.mark.parametrize("variant", ["strip", "balanced", "panel"])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-@pytest.mark.parametrize("s", [8, 64, 256])
+@pytest.mark.parametrize("s", [8, 64, 128, 256])
 def test_heavy_strip_variants(gpu, variant, precision, s):
     """Power-law rows: heavy strips are split into units whose partial sums
-    are combined by the fix-up pass (balanced) or processed whole (strip)."""
+    are combined by the fix-up pass (balanced) or processed whole (strip).
+    The fix-up takes a strip in slices of 128 rows (64 with FP32 values): one
+    or two slices at s = 128, two or four at s = 256."""
     A = synth.rmat(15, 16, seed=4)
     print(f"case rmat15 s={s} {precision} {variant}", flush=True)
     M = AbhsfMatrix(block_size=s, precision=precision, variant=variant)
