@@ -53,6 +53,7 @@ class Stats(C.Structure):
         ("stored_bytes", C.c_uint64), ("index_bytes", C.c_uint64),
         ("build_ms", C.c_double),
         ("variant", C.c_int32), ("tile_zs1", C.c_int32), ("work_items", C.c_int64),
+        ("max_strip_work", C.c_int64),
     ]
 
     def as_dict(self):
@@ -66,6 +67,7 @@ class Stats(C.Structure):
             "build_ms": float(self.build_ms),
             "variant": VARIANT_NAMES.get(int(self.variant), int(self.variant)),
             "tile_zs1": bool(self.tile_zs1), "work_items": int(self.work_items),
+            "max_strip_work": int(self.max_strip_work),
         }
 
 

@@ -108,8 +108,22 @@ void reset_storage(abhsf_handle *h, int64_t m, int64_t n, int64_t nnz)
 #endif
 int64_t balance_unit(const abhsf::Storage &S) { return std::max<int64_t>(ABHSF_UNIT_ENTRIES, 8 * (int64_t)S.s); }
 
+int64_t max_strip_work(const abhsf::Storage &S, hipStream_t st)
+{
+    if (S.M == 0) return 0;
+    std::vector<uint32_t> ep((size_t)S.M + 1);
+    std::vector<int64_t> w((size_t)S.M, 0);
+    for (int x = 0; x < 4; x++) {
+        ABHSF_HIP(hipMemcpyAsync(ep.data(), S.eptr[x].get(), ep.size() * 4, hipMemcpyDeviceToHost, st));
+        ABHSF_HIP(hipStreamSynchronize(st));
+        for (int64_t I = 0; I < S.M; I++) w[(size_t)I] += (int64_t)ep[(size_t)I + 1] - ep[(size_t)I];
+    }
+    return *std::max_element(w.begin(), w.end());
+}
+
 void finish_build(abhsf_handle *h)
 {
+    h->S.max_strip_work = max_strip_work(h->S, h->stream);
     const int tile_rows =
         h->opts.variant != ABHSF_VARIANT_TILED ? 0 : (h->S.val_bytes == 4 ? abhsf::kTileRowsComp : abhsf::kTileRows);
     if (h->opts.variant != ABHSF_VARIANT_STRIP)
@@ -292,6 +306,7 @@ abhsf_status abhsf_get_stats(const abhsf_handle *h, abhsf_stats *out)
         out->variant = h->opts.variant == ABHSF_VARIANT_AUTO ? ABHSF_VARIANT_BALANCED : h->opts.variant;
         out->tile_zs1 = S.tile_zs1 ? 1 : 0;
         out->work_items = h->opts.variant == ABHSF_VARIANT_STRIP ? S.M : S.n_light + S.n_units;
+        out->max_strip_work = S.max_strip_work;
     });
 }
 
@@ -303,9 +318,15 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
         ABHSF_REQUIRE(h->S.built, ABHSF_ERR_STATE, "matrix not built");
         ABHSF_REQUIRE(nrhs >= 1 && nrhs <= 64, ABHSF_ERR_UNSUPPORTED, "nrhs must be in [1, 64]");
         ABHSF_REQUIRE(ldx >= nrhs && ldy >= nrhs, ABHSF_ERR_INVALID, "leading dimension smaller than nrhs");
-        if (h->S.tiled && !(nrhs == 1 && ldx == 1 && ldy == 1)) {
+        // The SpMM kernel gives each strip to one wave: where one strip holds
+        // more than 1/4096 of the entries (power-law rows: R-MAT scale 22's
+        // first strip holds 1.2e6 of 6.5e7), that wave outlasts the rest of
+        // the launch (k = 8: 37.6 ms) and each right-hand side goes through
+        // the SpMV kernels instead, whose units split heavy strips (5.8 ms).
+        const bool serial_strip = h->S.max_strip_work > std::max<int64_t>(4096, h->S.nnz / 4096);
+        if ((h->S.tiled || serial_strip) && !(nrhs == 1 && ldx == 1 && ldy == 1)) {
             // tiles reorder the COO blocks, which the SpMM kernels walk strip by
-            // strip: each right-hand side goes through the tiled SpMV kernels,
+            // strip: each right-hand side goes through the (tiled) SpMV kernels,
             // gathered into and scattered from contiguous vectors the handle
             // keeps (allocated at the first such call)
             if (h->col_x.bytes() < (size_t)h->S.n * 8) h->col_x.alloc((size_t)h->S.n * 8);
@@ -315,6 +336,10 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
                 abhsf::strided_copy(X + j, ldx, cx, 1, h->S.n, h->stream);
                 if (beta != 0.0) abhsf::strided_copy(Y + j, ldy, cy, 1, h->S.m, h->stream);
                 abhsf::spmm(h->S.launch(), h->S.view(), 1, alpha, cx, 1, beta, cy, 1, h->stream);
+                if (h->built_symmetry != ABHSF_GENERAL && alpha != 0.0)  // the mirrored triangle
+                    abhsf::spmv_transposed(h->S.launch(), h->S.view(), 1,
+                                           h->built_symmetry == ABHSF_SKEW_SYMMETRIC ? -alpha : alpha, cx, 1, cy,
+                                           1, h->stream);
                 abhsf::strided_copy(cy, 1, Y + j, ldy, h->S.m, h->stream);
             }
             return;

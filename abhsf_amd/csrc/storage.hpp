@@ -29,6 +29,9 @@ struct Storage {
     bool tile_zs1 = false;  // tiles decode one COO entry per block in phase A (see Balance)
     int coo_zb = 16;
     DevBuf bal_coo_rng;  // tiled: [n_light][3] COO {first block, end block, first value} of each item
+    // entries of the largest strip (dense blocks: positions), for the SpMM
+    // dispatch: its kernel gives each strip to one wave
+    int64_t max_strip_work = 0;
     // footprints (bits) under both cost models
     uint64_t smf_preproc = 0, smf_msmf = 0;
     double build_ms = 0.0;

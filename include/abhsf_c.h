@@ -113,6 +113,9 @@ typedef struct abhsf_stats {
     int32_t tile_zs1;              /* tiled: 1 if tile COO decode takes one entry per block in its
                                       first phase (blocks of 3+ entries are 1 in 64 or more), else 0 */
     int64_t work_items;            /* single-RHS work items: light items + heavy-strip units */
+    int64_t max_strip_work;        /* entries of the largest strip (dense blocks: positions); above
+                                      max(4096, nnz / 4096) abhsf_spmm runs each right-hand side
+                                      through the SpMV kernels */
 } abhsf_stats;
 
 typedef struct abhsf_handle abhsf_handle;

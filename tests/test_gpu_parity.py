@@ -331,6 +331,35 @@ def test_heavy_strip_variants(gpu, variant, precision, s):
     assert relerr(dy.numpy(), ref) <= TOL
 
 
+@pytest.mark.parametrize("variant", ["strip", "balanced"])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("s", [64, 256])
+def test_spmm_power_law_rows(gpu, variant, precision, s):
+    """R-MAT: the first strip holds more than 1/4096 of the entries, so
+    abhsf_spmm runs each right-hand side through the SpMV kernels (which
+    split heavy strips) instead of giving the strip to one SpMM wave.
+    Integer values and X: exact, for an odd nrhs with ldx > nrhs too."""
+    A = synth.rmat(15, 16, seed=4)
+    rng = np.random.default_rng(s)
+    vi = (rng.integers(1, 9, A.nnz) * rng.choice([-1.0, 1.0], A.nnz)).astype(np.float64)
+    print(f"case spmm power-law s={s} {variant} {precision}", flush=True)
+    M = AbhsfMatrix(block_size=s, precision=precision, variant=variant)
+    M.from_csr(A.m, A.n, A.rowptr, A.col, vi)
+    st = M.stats()
+    assert st["max_strip_work"] > max(4096, A.nnz // 4096)
+    for nrhs, ldx, ldy in ((8, 8, 8), (13, 16, 15)):
+        X = rng.integers(-8, 9, (A.n, ldx)).astype(np.float64)
+        Y0 = rng.integers(-8, 9, (A.m, ldy)).astype(np.float64)
+        dX, dY = DeviceArray.from_numpy(X), DeviceArray.from_numpy(Y0)
+        M.spmm_async(nrhs, 1.0, dX, ldx, 2.0, dY, ldy)
+        M.synchronize()
+        got = dY.numpy()
+        for j in range(nrhs):
+            ref = pyoracle.csr_spmv(A.rowptr, A.col, vi, X[:, j].copy(), 1.0, 2.0, Y0[:, j].copy())
+            assert np.array_equal(got[:, j], ref), (nrhs, j)
+        assert np.array_equal(got[:, nrhs:], Y0[:, nrhs:])  # padding columns untouched
+
+
 @functools.lru_cache(maxsize=1)
 def two_full_rows(n=1 << 21, seed=0):
     """Rows 0 and 1 hold every column (2n entries: the strip that contains

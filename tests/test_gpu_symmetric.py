@@ -143,6 +143,39 @@ def test_symmetric_rejects_both_triangles_and_skew_diagonal(gpu):
     B.release()
 
 
+@pytest.mark.parametrize("skew", [False, True])
+@pytest.mark.parametrize("s", [16, 256])
+def test_symmetric_spmm_with_a_heavy_strip(gpu, skew, s):
+    """An arrow matrix stored as its upper triangle: row 0 holds every column,
+    so one strip holds more than 4096 entries and abhsf_spmm takes each
+    right-hand side through the SpMV kernels; the mirrored triangle must be
+    applied on that path too.  Integer data: exact."""
+    n = 20000
+    rng = np.random.default_rng(s)
+    c = np.arange(n)
+    r = np.concatenate([np.zeros(n, np.int64), np.arange(1, n)])
+    cc = np.concatenate([c, np.arange(1, n)])
+    v = (rng.integers(1, 9, r.size) * rng.choice([-1.0, 1.0], r.size)).astype(np.float64)
+    if skew:  # no diagonal
+        keep = r != cc
+        r, cc, v = r[keep], cc[keep], v[keep]
+    T = synth.from_triplets(n, n, r, cc, v)
+    print(f"case symmetric arrow spmm s={s} skew={skew}", flush=True)
+    M = AbhsfMatrix(block_size=s)
+    M.from_csr(T.m, T.n, T.rowptr, T.col, T.val, symmetry="skew" if skew else "symmetric")
+    st = M.stats()
+    assert st["max_strip_work"] > max(4096, st["nnz"] // 4096)
+    full = expand(T, skew)
+    nrhs = 5
+    X = rng.integers(-8, 9, (n, nrhs)).astype(np.float64)
+    dX, dY = DeviceArray.from_numpy(X), DeviceArray.zeros((n, nrhs))
+    M.spmm_async(nrhs, 1.0, dX, nrhs, 0.0, dY, nrhs)
+    M.synchronize()
+    got = dY.numpy()
+    for j in range(nrhs):
+        assert np.array_equal(got[:, j], pyoracle.csr_spmv(full.rowptr, full.col, full.val, X[:, j].copy())), j
+
+
 @pytest.mark.parametrize("case", ["dense", "random-column", "mtx_general", "mtx_symmetric", "mtx_skew",
                                   "random-column:tiled", "mtx_general:tiled"])
 def test_cpp_harness(gpu, case, tmp_path):
