@@ -327,20 +327,27 @@ abhsf_status abhsf_spmm(abhsf_handle *h, int32_t nrhs, double alpha, const doubl
         if ((h->S.tiled || serial_strip) && !(nrhs == 1 && ldx == 1 && ldy == 1)) {
             // tiles reorder the COO blocks, which the SpMM kernels walk strip by
             // strip: each right-hand side goes through the (tiled) SpMV kernels,
-            // gathered into and scattered from contiguous vectors the handle
+            // gathered into and scattered from contiguous columns the handle
             // keeps (allocated at the first such call)
-            if (h->col_x.bytes() < (size_t)h->S.n * 8) h->col_x.alloc((size_t)h->S.n * 8);
-            if (h->col_y.bytes() < (size_t)h->S.m * 8) h->col_y.alloc((size_t)h->S.m * 8);
+            // (blocks of up to col_block() columns move through a transposed tile)
+            const int cbw = abhsf::col_block();
+            const int64_t n = h->S.n, m = h->S.m;
+            if (h->col_x.bytes() < (size_t)n * 8 * cbw) h->col_x.alloc((size_t)n * 8 * cbw);
+            if (h->col_y.bytes() < (size_t)m * 8 * cbw) h->col_y.alloc((size_t)m * 8 * cbw);
             double *cx = h->col_x.get<double>(), *cy = h->col_y.get<double>();
-            for (int32_t j = 0; j < nrhs; j++) {
-                abhsf::strided_copy(X + j, ldx, cx, 1, h->S.n, h->stream);
-                if (beta != 0.0) abhsf::strided_copy(Y + j, ldy, cy, 1, h->S.m, h->stream);
-                abhsf::spmm(h->S.launch(), h->S.view(), 1, alpha, cx, 1, beta, cy, 1, h->stream);
-                if (h->built_symmetry != ABHSF_GENERAL && alpha != 0.0)  // the mirrored triangle
-                    abhsf::spmv_transposed(h->S.launch(), h->S.view(), 1,
-                                           h->built_symmetry == ABHSF_SKEW_SYMMETRIC ? -alpha : alpha, cx, 1, cy,
-                                           1, h->stream);
-                abhsf::strided_copy(cy, 1, Y + j, ldy, h->S.m, h->stream);
+            for (int32_t j0 = 0; j0 < nrhs; j0 += cbw) {
+                const int cols = std::min<int32_t>(cbw, nrhs - j0);
+                abhsf::gather_cols(X + j0, ldx, n, cols, cx, h->stream);
+                if (beta != 0.0) abhsf::gather_cols(Y + j0, ldy, m, cols, cy, h->stream);
+                for (int c = 0; c < cols; c++) {
+                    double *xc = cx + (int64_t)c * n, *yc = cy + (int64_t)c * m;
+                    abhsf::spmm(h->S.launch(), h->S.view(), 1, alpha, xc, 1, beta, yc, 1, h->stream);
+                    if (h->built_symmetry != ABHSF_GENERAL && alpha != 0.0)  // the mirrored triangle
+                        abhsf::spmv_transposed(h->S.launch(), h->S.view(), 1,
+                                               h->built_symmetry == ABHSF_SKEW_SYMMETRIC ? -alpha : alpha, xc, 1,
+                                               yc, 1, h->stream);
+                }
+                abhsf::scatter_cols(cy, m, cols, Y + j0, ldy, h->stream);
             }
             return;
         }

@@ -1685,20 +1685,62 @@ void spmv_transposed(const SpmvLaunch &L, const DevMat &A, int nrhs, double scal
 }
 
 namespace {
-__global__ __launch_bounds__(256) void k_strided_copy(const double *__restrict__ src, int64_t lds,
-                                                      double *__restrict__ dst, int64_t ldd, int64_t n)
+// Column blocks between a row-major n x ld array and contiguous columns
+// (column c of the block at dst + c * rows), through a transposed LDS tile so
+// that both sides are read and written in whole rows / runs: a workgroup
+// takes 256 rows of at most kColBlock columns.
+constexpr int kColBlock = 8;
+
+__global__ __launch_bounds__(256) void k_gather_cols(const double *__restrict__ src, int64_t lds, int64_t rows,
+                                                     int cols, double *__restrict__ dst)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dst[i * ldd] = src[i * lds];
+    __shared__ double t[kColBlock][257];
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    for (int e = threadIdx.x; e < 256 * cols; e += 256) {
+        const int rr = e / cols, c = e % cols;
+        if (r0 + rr < rows) t[c][rr] = src[(r0 + rr) * lds + c];
+    }
+    __syncthreads();
+    const int64_t r = r0 + threadIdx.x;
+    if (r < rows)
+        for (int c = 0; c < cols; c++) dst[(int64_t)c * rows + r] = t[c][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_scatter_cols(const double *__restrict__ src, int64_t rows, int cols,
+                                                      double *__restrict__ dst, int64_t ldd)
+{
+    __shared__ double t[kColBlock][257];
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int64_t r = r0 + threadIdx.x;
+    if (r < rows)
+        for (int c = 0; c < cols; c++) t[c][threadIdx.x] = src[(int64_t)c * rows + r];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256 * cols; e += 256) {
+        const int rr = e / cols, c = e % cols;
+        if (r0 + rr < rows) dst[(r0 + rr) * ldd + c] = t[c][rr];
+    }
 }
 }  // namespace
 
-void strided_copy(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t n, hipStream_t st)
+int col_block() { return kColBlock; }
+
+void gather_cols(const double *src, int64_t lds, int64_t rows, int cols, double *dst, hipStream_t st)
 {
-    if (n <= 0) return;
-    const int64_t nblk = (n + 255) / 256;
+    ABHSF_REQUIRE(cols >= 1 && cols <= kColBlock, ABHSF_ERR_INVALID, "column block too wide");
+    if (rows <= 0) return;
+    const int64_t nblk = (rows + 255) / 256;
     ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "vector too long");
-    hipLaunchKernelGGL(k_strided_copy, dim3((unsigned)nblk), dim3(256), 0, st, src, lds, dst, ldd, n);
+    hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)nblk), dim3(256), 0, st, src, lds, rows, cols, dst);
+    ABHSF_HIP(hipGetLastError());
+}
+
+void scatter_cols(const double *src, int64_t rows, int cols, double *dst, int64_t ldd, hipStream_t st)
+{
+    ABHSF_REQUIRE(cols >= 1 && cols <= kColBlock, ABHSF_ERR_INVALID, "column block too wide");
+    if (rows <= 0) return;
+    const int64_t nblk = (rows + 255) / 256;
+    ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "vector too long");
+    hipLaunchKernelGGL(k_scatter_cols, dim3((unsigned)nblk), dim3(256), 0, st, src, rows, cols, dst, ldd);
     ABHSF_HIP(hipGetLastError());
 }
 

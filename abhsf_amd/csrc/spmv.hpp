@@ -87,7 +87,11 @@ void spmv_transposed(const SpmvLaunch &L, const DevMat &A, int nrhs, double scal
                      double *Y, int64_t ldy, hipStream_t st);
 
 // dst[i * ldd] = src[i * lds] for i < n, enqueued on st.
-void strided_copy(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t n, hipStream_t st);
+// Columns [0, cols) of a row-major rows x ld array to contiguous columns (column
+// c at dst + c * rows) and back; cols <= col_block().
+int col_block();
+void gather_cols(const double *src, int64_t lds, int64_t rows, int cols, double *dst, hipStream_t st);
+void scatter_cols(const double *src, int64_t rows, int cols, double *dst, int64_t ldd, hipStream_t st);
 
 // Largest power-of-two rhs tile whose s x R accumulator fits the LDS budget.
 int max_rhs_tile(int s, bool comp);
