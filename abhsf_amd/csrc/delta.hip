@@ -145,10 +145,12 @@ __device__ inline uint32_t wave_incl_max_u32(uint32_t v)
 // scan) and the last lane of a run adds its sum to y atomically (one run per
 // row and step).  8 x 8 cache-block keys alternate rows inside each cache
 // block, so runs are one element long; their products go to a per-wave LDS
-// bin of kBandsD 8-row bands from the step's first band (rows further on take
-// a global add), and the bin's touched rows are added to y after the step
-// (R-MAT scale 22: one global add per element took 2.84 ms).
-constexpr int kBandsD = 4;
+// bin of kBandsD 8-row bands (rows further on take a global add), a window
+// that moves up to each step's first band: the bands it leaves are complete
+// and their touched rows are added to y then (R-MAT scale 22: one global add
+// per element took 2.84 ms, a flush of the bin after every step 0.52 ms).
+constexpr int kBandsD = 8;
+static_assert(kBandsD * 8 <= 64 && (kBandsD & (kBandsD - 1)) == 0, "one bin slot per lane, power of two bands");
 
 template <int CODEC>
 __global__ __launch_bounds__(256) void k_delta32_spmv(const uint32_t *__restrict__ da,
@@ -170,6 +172,23 @@ __global__ __launch_bounds__(256) void k_delta32_spmv(const uint32_t *__restrict
     uint64_t carry = base[c];
     uint32_t xo = xstart[c];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // cache-block keys: bin slot (band % kBandsD) * 8 + row % 8 holds the rows
+    // of one band of the window [wb, wb + kBandsD); flush_bins(lim) adds the
+    // bins of the window's bands below lim to y and clears them
+    uint64_t wb = 0;
+    bool have_wb = false;
+    auto flush_bins = [&](uint64_t lim) {
+        if (lane < kBandsD * 8) {
+            const uint64_t sb = (uint64_t)(lane >> 3);
+            const uint64_t b = wb + ((sb - (wb & (kBandsD - 1))) & (kBandsD - 1));
+            const double v = bin[lane];
+            if (b < lim && v != 0.0) {
+                atomicAdd(&y[b * 8 + (uint64_t)(lane & 7)], v);
+                bin[lane] = 0.0;
+            }
+        }
+        wave_sync();
+    };
 #pragma unroll 1
     for (int step = 0; step < kStepsD; step++) {
         const int64_t e0 = c * kDeltaChunk + (int64_t)step * kWaveD;
@@ -205,21 +224,21 @@ __global__ __launch_bounds__(256) void k_delta32_spmv(const uint32_t *__restrict
         if (!valid) row = ~0ull;
         double p = valid ? alpha * val[e] * x[col] : 0.0;
         if constexpr (CODEC == kDeltaU32Block) {
-            // lane 0 holds the step's first (smallest) row: e0 < nnz
+            // lane 0 holds the step's first (smallest) row: e0 < nnz.  Bands
+            // never decrease, so the bands before it are complete: their bins
+            // go to y and the window moves up to it
             const uint64_t band0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), 0)
                                     << 32 | (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, 0)) >> 3;
-            const uint64_t db = (row >> 3) - band0;
-            if (valid) {
-                if (db < (uint64_t)kBandsD) atomicAdd(&bin[db * 8 + (row & 7ull)], p);
-                else atomicAdd(&y[row], p);
+            if (!have_wb) {
+                wb = band0;
+                have_wb = true;
+            } else if (band0 > wb) {
+                flush_bins(band0);
+                wb = band0;
             }
-            wave_sync();
-            if (lane < kBandsD * 8) {
-                const double v = bin[lane];
-                if (v != 0.0) {
-                    atomicAdd(&y[(band0 + (uint64_t)(lane >> 3)) * 8 + (uint64_t)(lane & 7)], v);
-                    bin[lane] = 0.0;
-                }
+            if (valid) {
+                if ((row >> 3) - wb < (uint64_t)kBandsD) atomicAdd(&bin[((row >> 3) & (kBandsD - 1)) * 8 + (row & 7ull)], p);
+                else atomicAdd(&y[row], p);
             }
             wave_sync();
             continue;
@@ -235,6 +254,9 @@ __global__ __launch_bounds__(256) void k_delta32_spmv(const uint32_t *__restrict
         seg_step<DPP_ROW_BCAST15, 0xa>(p, f);
         seg_step<DPP_ROW_BCAST31, 0xc>(p, f);
         if (valid && (lane == kWaveD - 1 || next_head)) atomicAdd(&y[row], p);
+    }
+    if constexpr (CODEC == kDeltaU32Block) {
+        if (have_wb) flush_bins(~0ull);  // the window's last bands
     }
 }
 
