@@ -230,32 +230,39 @@ def main_batch(args, log):
         f"mean regret {report['mean_regret_pct']}")
 
 
-def cross_validate(Xtr, ytr, ntr, nc, grid, folds=5, members=5, seed=0, log=print):
-    """k-fold cross-validation by matrix over the training set: for every
-    (steps, lr) of the grid, the mean regret of the held-in model's pick on
-    each held-out fold's matrices (labels ytr are log times per entry, so
-    time ratios within a matrix are exp of label differences).  Returns
-    {(steps, lr): (mean regret %, its standard error)} and the choice by the
+def cross_validate(Xtr, ytr, ntr, nc, grid, folds=5, repeats=3, members=5, seed=0, log=print):
+    """k-fold cross-validation by matrix over the training set, repeated over
+    `repeats` fold assignments: for every (steps, lr) of the grid, each
+    matrix's regret of the held-in model's pick when it is held out (labels
+    ytr are log times per entry, so time ratios within a matrix are exp of
+    label differences), averaged over the repeats.  Returns {(steps, lr):
+    (mean regret % over matrices, its standard error)} and the choice by the
     one-standard-error rule: the fewest steps whose mean regret is within one
     standard error of the lowest (the simplest model the folds cannot tell
-    apart from the best)."""
+    apart from the best).  One assignment alone moved that choice between
+    runs (its standard errors overlap widely); the repeats steady it."""
     nmat = len(ntr)
-    order = np.random.default_rng(seed).permutation(nmat)
-    fold_of = np.empty(nmat, dtype=np.int64)
-    fold_of[order] = np.arange(nmat) % folds
     Y = ytr.reshape(nmat, nc)
     X = Xtr.reshape(nmat, nc, -1)
+    assign = []
+    for r in range(repeats):
+        order = np.random.default_rng(seed + r).permutation(nmat)
+        fold_of = np.empty(nmat, dtype=np.int64)
+        fold_of[order] = np.arange(nmat) % folds
+        assign.append(fold_of)
     res = {}
     for steps, lr in grid:
-        regrets = []
-        for f in range(folds):
-            tr, te = np.flatnonzero(fold_of != f), np.flatnonzero(fold_of == f)
-            model, _ = tuner.train(X[tr].reshape(-1, X.shape[-1]), Y[tr].reshape(-1), steps=steps, lr=lr,
-                                   members=members)
-            for i in te:
-                k = int(np.argmin(model.predict(X[i])))
-                regrets.append((math.exp(Y[i, k] - Y[i].min()) - 1.0) * 100.0)
-        res[(steps, lr)] = (float(np.mean(regrets)), float(np.std(regrets) / math.sqrt(len(regrets))))
+        per = np.zeros(nmat)
+        for rep, fold_of in enumerate(assign):
+            log(f"cv steps={steps} lr={lr}: fold assignment {rep + 1}/{repeats}")
+            for f in range(folds):
+                tr, te = np.flatnonzero(fold_of != f), np.flatnonzero(fold_of == f)
+                model, _ = tuner.train(X[tr].reshape(-1, X.shape[-1]), Y[tr].reshape(-1), steps=steps, lr=lr,
+                                       members=members)
+                for i in te:
+                    k = int(np.argmin(model.predict(X[i])))
+                    per[i] += (math.exp(Y[i, k] - Y[i].min()) - 1.0) * 100.0 / repeats
+        res[(steps, lr)] = (float(np.mean(per)), float(np.std(per) / math.sqrt(nmat)))
         log(f"cv steps={steps} lr={lr}: mean regret {res[(steps, lr)][0]:.2f}% (se {res[(steps, lr)][1]:.2f})")
     best = min(res, key=lambda k: res[k][0])
     bound = res[best][0] + res[best][1]
@@ -271,7 +278,7 @@ def main(argv=None):
     # the batch set is 8x larger and keeps the longer schedule
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--lr", type=float, default=None)
-    ap.add_argument("--members", type=int, default=5)
+    ap.add_argument("--members", type=int, default=None)
     ap.add_argument("--mode", choices=["latency", "batch"], default="latency")
     ap.add_argument("--cv", action="store_true",
                     help="latency: choose steps and lr by 5-fold cross-validation over the training matrices")
@@ -281,7 +288,9 @@ def main(argv=None):
     if args.steps is None:
         args.steps = 4000 if args.mode == "batch" else 700
     if args.lr is None:
-        args.lr = 2e-3 if args.mode == "batch" else 1e-2
+        args.lr = 2e-3 if args.mode == "batch" else 3e-3
+    if args.members is None:  # latency: 10 members cross-validated better than 5
+        args.members = 5 if args.mode == "batch" else 10
     os.makedirs(args.out, exist_ok=True)
 
     def log(msg):
