@@ -390,12 +390,14 @@ SpmvLaunch Storage::launch() const
     L.bal.pack = bal_pack.get<uint32_t>();
     L.bal.units = bal_units.get<uint32_t>();
     L.bal.heavy = bal_heavy.get<uint32_t>();
+    L.bal.fix = n_fix ? bal_heavy.get<uint32_t>() + 3 * n_heavy : nullptr;
     L.bal.part = bal_part.get<double>();
     L.bal.order = bal_order.get<uint32_t>();
     L.bal.n_light = n_light;
     L.bal.n_units = n_units;
     L.bal.n_heavy = n_heavy;
     L.bal.n_slots = n_slots;
+    L.bal.n_fix = n_fix;
     L.bal.tiled = tiled;
     L.bal.tile_zs1 = tile_zs1;
     L.bal.coo_rng = bal_coo_rng.get<uint32_t>();
@@ -458,7 +460,7 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels, int tile_r
     bal_part.release();
     bal_order.release();
     bal_coo_rng.release();
-    n_light = n_units = n_heavy = n_slots = 0;
+    n_light = n_units = n_heavy = n_slots = n_fix = 0;
     tile_zs1 = false;
     if (unit <= 0 || M == 0) return;
     ABHSF_REQUIRE(panels == 1 || panels == kXcdCount, ABHSF_ERR_INVALID, "column panels must match the XCD count");
@@ -738,6 +740,16 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels, int tile_r
     n_units = (int64_t)(units.size() / 4);
     n_heavy = (int64_t)(heavy.size() / 3);
     ABHSF_REQUIRE(n_light + n_units < (int64_t)kNoItem, ABHSF_ERR_UNSUPPORTED, "too many work items");
+    // fix-up workgroups: row slices of every heavy strip, narrower for the
+    // strips with more units (stored after the heavy strips)
+    std::vector<uint32_t> fix;
+    for (int64_t h = 0; h < n_heavy; h++) {
+        const int rs = fix_rows(s, heavy[(size_t)h * 3 + 2], val_bytes == 4);
+        for (int r0 = 0; r0 < s; r0 += rs) fix.insert(fix.end(), {(uint32_t)h, (uint32_t)r0 | ((uint32_t)rs << 16)});
+    }
+    n_fix = (int64_t)(fix.size() / 2);
+    ABHSF_REQUIRE(n_fix < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many heavy strips");
+    heavy.insert(heavy.end(), fix.begin(), fix.end());
     std::vector<uint32_t> order;
     if (panels > 1) {
         std::vector<std::vector<uint32_t>> lists((size_t)panels);

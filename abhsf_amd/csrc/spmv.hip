@@ -950,6 +950,7 @@ static_assert(kFixThreads >= 256, "at least one thread per row of a 256-row stri
 #endif
 template <bool COMP>
 constexpr int kFixChains = COMP ? ABHSF_FIX_CHAINS / 2 : ABHSF_FIX_CHAINS;
+static_assert(kFixChains<true> >= 4, "a quarter round of the fix-up takes one chain at least");
 // Rows per fix-up workgroup: a strip of s > kFixRows rows is summed by s /
 // kFixRows workgroups, kFixThreads / kFixRows threads per row.  R-MAT scale
 // 22 at s = 256 (heaviest strip 585 units), fix-up with 256 / 128 / 64-row
@@ -957,6 +958,14 @@ constexpr int kFixChains = COMP ? ABHSF_FIX_CHAINS / 2 : ABHSF_FIX_CHAINS;
 template <bool COMP>
 constexpr int kFixRows = COMP ? 64 : 128;
 static_assert(kFixThreads % 128 == 0, "whole threads per row");
+// A heavy strip whose units would leave more than kFixUnits of them to each
+// thread gets narrower slices (rows halved down to kFixRowsMin, threads per
+// row doubled): the strips with the most units set the pass's length.
+#ifndef ABHSF_FIX_UNITS
+#define ABHSF_FIX_UNITS 32
+#endif
+constexpr int kFixUnits = ABHSF_FIX_UNITS;
+constexpr int kFixRowsMin = 16;
 
 template <bool COMP>
 __device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
@@ -968,6 +977,22 @@ __device__ inline void fix_add(double &hi, double &lo, double v, double vlo)
         lo += vlo + e2;
     } else {
         hi += v;
+    }
+}
+
+// One round of the fix-up's chains 0 .. W - 1 (units u + j * L; a chain past
+// nu reads unit l and adds 0).
+template <bool COMP, int W>
+__device__ inline void fix_round(const double *__restrict__ part, size_t stride, uint32_t u0, uint32_t u, uint32_t l,
+                                 uint32_t L, uint32_t nu, int s, int r, double *hi, double *lo)
+{
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        const uint32_t uj = u + (uint32_t)j * L;
+        const bool ok = uj < nu;
+        const double *p = part + (size_t)(u0 + (ok ? uj : l)) * stride;
+        const double v = p[r], vlo = COMP ? p[s + r] : 0.0;
+        fix_add<COMP>(hi[j], lo[j], ok ? v : 0.0, ok ? vlo : 0.0);
     }
 }
 
@@ -984,17 +1009,21 @@ __device__ inline void fixup_strip(int s, int rs, int r0, int64_t m, int64_t I, 
     double hi[kFixChains<COMP>], lo[kFixChains<COMP>];
 #pragma unroll
     for (int j = 0; j < kFixChains<COMP>; j++) hi[j] = lo[j] = 0.0;
+    // chain j takes units u + j * L: full rounds of kFixChains loads, then
+    // one round of kFixChains, half or a quarter of them, whichever covers
+    // the units left.  Its chains past nu read unit l again and add 0: no
+    // branch, so a round's loads are all in flight together (a branch per
+    // chain serialised them: R-MAT scale 22 s = 256 with 128-row slices,
+    // SpMV 504 -> 584 us).
+    constexpr int C = kFixChains<COMP>;
     uint32_t u = l;
-    for (; u + (uint32_t)(kFixChains<COMP> - 1) * L < nu; u += (uint32_t)kFixChains<COMP> * L) {
-#pragma unroll
-        for (int j = 0; j < kFixChains<COMP>; j++) {
-            const double *p = part + (size_t)(u0 + u + (uint32_t)j * L) * stride;
-            fix_add<COMP>(hi[j], lo[j], p[r], COMP ? p[s + r] : 0.0);
-        }
-    }
-    for (; u < nu; u += L) {
-        const double *p = part + (size_t)(u0 + u) * stride;
-        fix_add<COMP>(hi[0], lo[0], p[r], COMP ? p[s + r] : 0.0);
+    for (; u + (uint32_t)(C - 1) * L < nu; u += (uint32_t)C * L)
+        fix_round<COMP, C>(part, stride, u0, u, l, L, nu, s, r, hi, lo);
+    if (u < nu) {
+        const uint32_t left = (nu - u + L - 1) / L;  // 1 .. C - 1
+        if (left > (uint32_t)C / 2) fix_round<COMP, C>(part, stride, u0, u, l, L, nu, s, r, hi, lo);
+        else if (left > (uint32_t)C / 4) fix_round<COMP, C / 2>(part, stride, u0, u, l, L, nu, s, r, hi, lo);
+        else fix_round<COMP, C / 4>(part, stride, u0, u, l, L, nu, s, r, hi, lo);
     }
 #pragma unroll
     for (int j = 1; j < kFixChains<COMP>; j++) fix_add<COMP>(hi[0], lo[0], hi[j], lo[j]);
@@ -1046,10 +1075,9 @@ __global__ __launch_bounds__(kFixThreads) void k_spmv_fixup(int s, int64_t m, do
                                                             double beta, Balance B)
 {
     __shared__ double sh[kFixThreads * (COMP ? 2 : 1)];
-    // strips of more than kFixRows rows: one workgroup per kFixRows-row slice
-    const int rs = s < kFixRows<COMP> ? s : kFixRows<COMP>, slices = s / rs;
-    const int64_t h = blockIdx.x / slices;
-    const int r0 = (int)(blockIdx.x % slices) * rs;
+    // workgroup b sums rows [r0, r0 + rs) of heavy strip h (fix_rows)
+    const uint32_t h = B.fix[2 * (size_t)blockIdx.x], w = B.fix[2 * (size_t)blockIdx.x + 1];
+    const int r0 = (int)(w & 0xFFFFu), rs = (int)(w >> 16);
     fixup_strip<COMP>(s, rs, r0, m, B.heavy[h * 3], B.part, B.heavy[h * 3 + 1], B.heavy[h * 3 + 2], y, alpha, beta,
                       sh);
 }
@@ -1580,10 +1608,9 @@ void launch_spmv(const DevMat &A, const Balance &B, bool coo_single, const doubl
         launch_strip<VT, BT, IT, RT, COMP, false>(A, b, nblk, coo_single, x, y, alpha, beta, st);
     ABHSF_HIP(hipGetLastError());
     if (b.n_heavy) {
-        const int64_t slices = A.s > kFixRows<COMP> ? A.s / kFixRows<COMP> : 1;
-        ABHSF_REQUIRE(b.n_heavy * slices < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many heavy strips");
-        hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)(b.n_heavy * slices)), dim3(kFixThreads), 0, st,
-                           A.s, A.m, y, alpha, beta, b);
+        ABHSF_REQUIRE(b.n_fix > 0 && b.n_fix < (1ll << 31), ABHSF_ERR_STATE, "fix-up table missing");
+        hipLaunchKernelGGL((k_spmv_fixup<COMP>), dim3((unsigned)b.n_fix), dim3(kFixThreads), 0, st, A.s, A.m, y,
+                           alpha, beta, b);
         ABHSF_HIP(hipGetLastError());
     }
 }
@@ -1628,6 +1655,13 @@ void launch_v(const SpmvLaunch &L, const DevMat &A, int R, const double *X, int6
 }
 
 }  // namespace
+
+int fix_rows(int s, uint32_t nu, bool comp)
+{
+    int rs = std::min(s, comp ? kFixRows<true> : kFixRows<false>);
+    while (rs > kFixRowsMin && (uint64_t)nu * (uint64_t)rs > (uint64_t)kFixUnits * (uint64_t)kFixThreads) rs /= 2;
+    return rs;
+}
 
 int batch_code(int bcol_bytes, int s) { return (bcol_bytes == 4 ? 3 : 0) + (s <= 8 ? 0 : (s <= 16 ? 1 : 2)); }
 

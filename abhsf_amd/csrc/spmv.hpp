@@ -15,12 +15,13 @@ struct Balance {
                                       // (null: one strip per light item)
     const uint32_t *units = nullptr;  // [n_units][4] {scheme | strip << 2, b0, b1, value offset of b0}
     const uint32_t *heavy = nullptr;  // [n_heavy][3] {strip, first unit, unit count}
+    const uint32_t *fix = nullptr;    // [n_fix][2] fix-up workgroups {heavy index, first row | rows << 16}
     double *part = nullptr;           // [n_units][s] (x2 when compensated)
     // Column-panel variant: slot table, 4 slots (waves) per workgroup; slot
     // 4b + w holds the item of wave w of workgroup b, or kNoItem.  Workgroup b
     // runs on XCD b % kXcdCount, so items are grouped by panel = XCD.
     const uint32_t *order = nullptr;  // [n_slots]
-    int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0;
+    int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0, n_fix = 0;
     // Tiled variant: light item k holds the strips light[pack[k] .. pack[k+1])
     // (a tile of COO-only strips, or one strip), accumulated together (their
     // rows (position << k) + lrow; kTileRows rows at most, kTileRowsComp with
@@ -86,12 +87,15 @@ void batch_spmv(const BatchEntry *dE, const uint32_t *item_mat, int64_t G64, int
 void spmv_transposed(const SpmvLaunch &L, const DevMat &A, int nrhs, double scale, const double *X, int64_t ldx,
                      double *Y, int64_t ldy, hipStream_t st);
 
-// dst[i * ldd] = src[i * lds] for i < n, enqueued on st.
 // Columns [0, cols) of a row-major rows x ld array to contiguous columns (column
 // c at dst + c * rows) and back; cols <= col_block().
 int col_block();
 void gather_cols(const double *src, int64_t lds, int64_t rows, int cols, double *dst, hipStream_t st);
 void scatter_cols(const double *src, int64_t rows, int cols, double *dst, int64_t ldd, hipStream_t st);
+
+// Rows per fix-up workgroup for a heavy strip of nu units (s-row strips;
+// comp: FP32 values, compensated partials): a power of two that divides s.
+int fix_rows(int s, uint32_t nu, bool comp);
 
 // Largest power-of-two rhs tile whose s x R accumulator fits the LDS budget.
 int max_rhs_tile(int s, bool comp);

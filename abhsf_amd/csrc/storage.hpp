@@ -19,9 +19,10 @@ struct Storage {
     DevBuf sptr[4], eptr[4], bcol[4], z1[4], val[4];
     DevBuf coo_idx, csr_rowptr, csr_col, bm_words;
     // work decomposition of the single-RHS kernel (heavy strips split in units;
-    // with column panels, the slot table places each item on its panel's XCD)
+    // with column panels, the slot table places each item on its panel's XCD;
+    // bal_heavy holds the heavy strips, then the fix-up workgroups)
     DevBuf bal_light, bal_pack, bal_units, bal_heavy, bal_part, bal_order;
-    int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0;
+    int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0, n_fix = 0;
     // tiled variant: light items are tiles of consecutive strips whose COO
     // blocks are stored in (block column, strip) order; COO z1 holds z - 1 in
     // its low coo_zb bits and the strip within the tile above them

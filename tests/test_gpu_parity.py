@@ -314,8 +314,8 @@ def test_repeated_spmv_bitwise_identical(gpu, variant):
 def test_heavy_strip_variants(gpu, variant, precision, s):
     """Power-law rows: heavy strips are split into units whose partial sums
     are combined by the fix-up pass (balanced) or processed whole (strip).
-    The fix-up takes a strip in slices of 128 rows (64 with FP32 values): one
-    or two slices at s = 128, two or four at s = 256."""
+    The fix-up takes a strip in slices of at most 128 rows (64 with FP32
+    values): one or two slices at s = 128, two or four at s = 256."""
     A = synth.rmat(15, 16, seed=4)
     print(f"case rmat15 s={s} {precision} {variant}", flush=True)
     M = AbhsfMatrix(block_size=s, precision=precision, variant=variant)
@@ -329,6 +329,53 @@ def test_heavy_strip_variants(gpu, variant, precision, s):
     vals = A.val if precision == "f64" else A.val.astype(np.float32).astype(np.float64)
     ref = pyoracle.csr_spmv(A.rowptr, A.col, vals, x, 2.0, 0.5, y0)
     assert relerr(dy.numpy(), ref) <= TOL
+
+
+def strips_of_many_units(seed):
+    """Rows 0-63 full over 10,000 columns (about 312 units of 2,048 entries),
+    rows 64-127 over 1,500 (47 units), rows 128-191 over 320 (10 units), the
+    other rows tridiagonal: at s = 64 three heavy strips of very different
+    unit counts, at s = 256 one of about 370."""
+    n = 12000
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for r0, w in ((0, 10000), (64, 1500), (128, 320)):
+        rr = np.repeat(np.arange(r0, r0 + 64), w)
+        rows.append(rr)
+        cols.append((np.tile(np.arange(w), 64) * 7 + rr) % n)
+    t = np.arange(192, n)
+    for d in (-1, 0, 1):
+        ok = (t + d >= 0) & (t + d < n)
+        rows.append(t[ok])
+        cols.append(t[ok] + d)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    return synth.from_triplets(n, n, r, c, rng.uniform(-1, 1, r.size), "many_units")
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("s", [64, 256])
+def test_fixup_slices_by_unit_count(gpu, precision, s):
+    """The fix-up narrows a heavy strip's row slices (down to 16 rows, more
+    threads per row) when it has many units, and each thread ends with one
+    round of all, half or a quarter of its chains: strips of about 370, 312,
+    47 and 10 units cover the narrow slices and every last-round width."""
+    A = strips_of_many_units(11)
+    print(f"case many_units s={s} {precision}", flush=True)
+    M = AbhsfMatrix(block_size=s, precision=precision, variant="balanced")
+    M.from_csr(A.m, A.n, A.rowptr, A.col, A.val)
+    x = np.random.default_rng(12).uniform(-1, 1, A.n)
+    y0 = np.random.default_rng(13).uniform(-1, 1, A.m)
+    dx = DeviceArray.from_numpy(x)
+    vals = A.val if precision == "f64" else A.val.astype(np.float32).astype(np.float64)
+    ref = pyoracle.csr_spmv(A.rowptr, A.col, vals, x, 1.5, -0.5, y0)
+    outs = []
+    for _ in range(2):
+        dy = DeviceArray.from_numpy(y0)
+        M.spmv_async(1.5, dx, -0.5, dy)
+        M.synchronize()
+        outs.append(dy.numpy())
+    assert relerr(outs[0], ref) <= TOL
+    assert np.array_equal(outs[0], outs[1]), "repeated SpMV differs bit for bit"
 
 
 @pytest.mark.parametrize("variant", ["strip", "balanced"])
