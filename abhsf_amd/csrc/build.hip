@@ -756,11 +756,12 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels, int tile_r
         for (int64_t i = 0; i < n_light; i++) lists[item_panel[i]].push_back((uint32_t)i);
         for (int64_t u = 0; u < n_units; u++) lists[unit_panel[u]].push_back((uint32_t)(n_light + u));
         size_t G = 0;
-        for (auto &l : lists) G = std::max(G, (l.size() + 3) / 4);
-        order.assign(G * (size_t)panels * 4, kNoItem);
+        const size_t w = (size_t)kStripWaves;
+        for (auto &l : lists) G = std::max(G, (l.size() + w - 1) / w);
+        order.assign(G * (size_t)panels * w, kNoItem);
         for (int p = 0; p < panels; p++)
             for (size_t j = 0; j < lists[p].size(); j++)
-                order[((j / 4) * (size_t)panels + (size_t)p) * 4 + j % 4] = lists[p][j];
+                order[((j / w) * (size_t)panels + (size_t)p) * w + j % w] = lists[p][j];
         n_slots = (int64_t)order.size();
     }
     bal_light.alloc(light.size() * 4);
@@ -817,7 +818,7 @@ void Storage::build_balance(int64_t unit, hipStream_t st, int panels, int tile_r
             for (int64_t i = 0; i < ntiles; i++) ord.push_back((uint32_t)i);
             for (uint32_t u : uo) ord.push_back((uint32_t)(n_light + u));
             for (int64_t i = ntiles; i < n_light; i++) ord.push_back((uint32_t)i);
-            while (ord.size() % 4) ord.push_back(kNoItem);
+            while (ord.size() % kStripWaves) ord.push_back(kNoItem);
             bal_order.alloc(ord.size() * 4);
             ABHSF_HIP(hipMemcpyAsync(bal_order.get(), ord.data(), ord.size() * 4, hipMemcpyHostToDevice, st));
             n_slots = (int64_t)ord.size();

@@ -655,10 +655,11 @@ __device__ inline void spmv_item(const DevMat &A, const Balance &B, int64_t item
 // units u of heavy strips: one scheme group's block range, whose partial sums
 // go to part[u] and are combined by k_spmv_fixup.
 template <typename VT, typename BT, typename IT, typename RT, bool COMP, int ZS, int CH, bool TILED>
-__global__ __launch_bounds__(256) void k_spmv_strip(DevMat A, const double *__restrict__ x, double *__restrict__ y,
-                                                    double alpha, double beta, Balance B)
+__global__ __launch_bounds__(64 * kStripWaves) void k_spmv_strip(DevMat A, const double *__restrict__ x,
+                                                                 double *__restrict__ y, double alpha, double beta,
+                                                                 Balance B)
 {
-    constexpr int kWaves = 4;
+    constexpr int kWaves = kStripWaves;
     // s rows, or a tile's (kTileRows; kTileRowsComp compensated, lo half at +256)
     static_assert(kTileRowsComp == 256 && kTileRows >= 256, "accumulator layout");
     __shared__ double s_acc[kWaves][COMP ? 512 : kTileRows];
@@ -1573,20 +1574,20 @@ void launch_strip(const DevMat &A, const Balance &b, int64_t nblk, bool coo_sing
     // 0.624 -> 0.606 ms).
     if constexpr (!TILED) {
         if (coo_single) {
-            hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 8, false>), dim3((unsigned)nblk), dim3(256),
-                               0, st, A, x, y, alpha, beta, b);
+            hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 8, false>), dim3((unsigned)nblk),
+                               dim3(64 * kStripWaves), 0, st, A, x, y, alpha, beta, b);
             return;
         }
     }
     if constexpr (TILED) {
         if (b.tile_zs1) {
-            hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 4, true>), dim3((unsigned)nblk), dim3(256), 0,
-                               st, A, x, y, alpha, beta, b);
+            hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 1, 4, true>), dim3((unsigned)nblk),
+                               dim3(64 * kStripWaves), 0, st, A, x, y, alpha, beta, b);
             return;
         }
     }
-    hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 2, 4, TILED>), dim3((unsigned)nblk), dim3(256), 0, st, A,
-                       x, y, alpha, beta, b);
+    hipLaunchKernelGGL((k_spmv_strip<VT, BT, IT, RT, COMP, 2, 4, TILED>), dim3((unsigned)nblk), dim3(64 * kStripWaves),
+                       0, st, A, x, y, alpha, beta, b);
 }
 
 template <typename VT, typename BT, typename IT, typename RT, bool COMP>
@@ -1597,7 +1598,7 @@ void launch_spmv(const DevMat &A, const Balance &B, bool coo_single, const doubl
     Balance b = B;
     if (!b.light && b.n_units == 0) b.n_light = A.M;  // one item per strip
     const int64_t nitems = b.n_light + b.n_units;
-    const int64_t nblk = b.order ? b.n_slots / 4 : (nitems + 3) / 4;
+    const int64_t nblk = b.order ? b.n_slots / kStripWaves : (nitems + kStripWaves - 1) / kStripWaves;
     ABHSF_REQUIRE(nblk < (1ll << 31), ABHSF_ERR_UNSUPPORTED, "too many strips");
     // tiled matrices have their own kernels (each carries one light-item
     // path); the choice is made once, here: a tiled storage read by an untiled

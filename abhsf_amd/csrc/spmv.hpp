@@ -17,9 +17,10 @@ struct Balance {
     const uint32_t *heavy = nullptr;  // [n_heavy][3] {strip, first unit, unit count}
     const uint32_t *fix = nullptr;    // [n_fix][2] fix-up workgroups {heavy index, first row | rows << 16}
     double *part = nullptr;           // [n_units][s] (x2 when compensated)
-    // Column-panel variant: slot table, 4 slots (waves) per workgroup; slot
-    // 4b + w holds the item of wave w of workgroup b, or kNoItem.  Workgroup b
-    // runs on XCD b % kXcdCount, so items are grouped by panel = XCD.
+    // Column-panel variant: slot table, kStripWaves slots (waves) per
+    // workgroup; slot kStripWaves * b + w holds the item of wave w of
+    // workgroup b, or kNoItem.  Workgroup b runs on XCD b % kXcdCount, so
+    // items are grouped by panel = XCD.
     const uint32_t *order = nullptr;  // [n_slots]
     int64_t n_light = 0, n_units = 0, n_heavy = 0, n_slots = 0, n_fix = 0;
     // Tiled variant: light item k holds the strips light[pack[k] .. pack[k+1])
@@ -46,6 +47,13 @@ constexpr int kTileRowsComp = 256;              // with FP32 values (two accumul
 constexpr int64_t kTileWork = ABHSF_TILE_WORK;  // entries per tile at most
 
 constexpr int kXcdCount = 8;            // XCDs of an MI355X; workgroups go round-robin
+// Waves per workgroup of the single-RHS strip kernel.  Each wave runs its own
+// items and holds its own LDS, but a workgroup's LDS is freed only when its
+// last wave ends: with 4 waves R-MAT scale 22 kept about 5,000 of 8,192 wave
+// slots busy.  2 waves: R-MAT (256, tiled) 490 -> 472 us, block_mix (32,
+// tiled) 455 -> 448 us; 1 wave is no faster and loses the column panels'
+// XCD grouping of 4-slot tables.
+constexpr int kStripWaves = 2;
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 
 struct SpmvLaunch {
